@@ -1,0 +1,475 @@
+// hmg_cell.hpp -- per-cell arithmetic of the erosion hot path, shared by
+// every kernel (plain and fused) so that they compute bit-identical values.
+//
+// Build contract: compiled with -ffp-contract=off, correctly rounded FP32
+// division and sqrt, denormals kept.  Each source operation below is one
+// IEEE-754 FP32 operation; the only fused multiply-adds are explicit fmaf
+// calls in the project-written sin/cos.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace hmg
+{
+
+// ----------------------------------------------------------------- storage
+
+// Storage codecs for the water, sediment and flux planes.  Values are
+// rounded to nearest even when stored; all arithmetic stays FP32.
+struct StF32
+{
+  using T = float;
+  static __device__ __forceinline__ float ld(T x) { return x; }
+  static __device__ __forceinline__ T     st(float x) { return x; }
+};
+
+struct StBF16
+{
+  using T = uint16_t;
+  static __device__ __forceinline__ float ld(T x)
+  {
+    return __uint_as_float(((uint32_t)x) << 16);
+  }
+  static __device__ __forceinline__ T st(float x)
+  {
+    uint32_t b = __float_as_uint(x);
+    if ((b & 0x7f800000u) == 0x7f800000u) return (T)((b >> 16) | ((b & 0xffffu) ? 0x40u : 0u));
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (T)(b >> 16);
+  }
+};
+
+struct StF16
+{
+  using T = _Float16;
+  static __device__ __forceinline__ float ld(T x) { return (float)x; }
+  static __device__ __forceinline__ T     st(float x) { return (_Float16)x; }
+};
+
+// Rounds a value to what the storage type would hold.
+template <class S>
+__device__ __forceinline__ float rnd(float x)
+{
+  return S::ld(S::st(x));
+}
+
+// Four flux components stored together (fl, fr, ft, fb).
+template <class S>
+struct alignas(4 * sizeof(typename S::T)) Flux4
+{
+  typename S::T l, r, t, b;
+};
+
+// ----------------------------------------------------- sin/cos and hashing
+
+// Project-written sin/cos: Cody-Waite reduction by pi/2 with a three-part
+// constant (fmaf at each step), then Taylor polynomials of sin (to r^9) and
+// cos (to r^10) by Horner with fmaf.  The reference calls OpenCL sin/cos
+// under -cl-fast-relaxed-math (HighMap/src/gpu_opencl/kernels/
+// _common_rand.cl:15,56), whose results are vendor-defined.
+__device__ __forceinline__ float reduce_pio2(float x, int &q)
+{
+  float k = __builtin_rintf(x * 0x1.45f306p-1f);
+  float r = fmaf(-k, 0x1.921fb6p+0f, x);
+  r = fmaf(-k, -0x1.777a5cp-25f, r);
+  r = fmaf(-k, -0x1.ee59dap-50f, r);
+  q = ((int)k) & 3;
+  return r;
+}
+
+__device__ __forceinline__ float sin_poly(float r)
+{
+  float r2 = r * r;
+  float p = fmaf(r2, 1.f / 362880.f, -1.f / 5040.f);
+  p = fmaf(r2, p, 1.f / 120.f);
+  p = fmaf(r2, p, -1.f / 6.f);
+  return fmaf(r * r2, p, r);
+}
+
+__device__ __forceinline__ float cos_poly(float r)
+{
+  float r2 = r * r;
+  float p = fmaf(r2, -1.f / 3628800.f, 1.f / 40320.f);
+  p = fmaf(r2, p, -1.f / 720.f);
+  p = fmaf(r2, p, 1.f / 24.f);
+  p = fmaf(r2, p, -0.5f);
+  return fmaf(r2, p, 1.f);
+}
+
+__device__ __forceinline__ float hm_sin(float x)
+{
+  int   q;
+  float r = reduce_pio2(x, q);
+  float s = sin_poly(r), c = cos_poly(r);
+  float v = (q & 1) ? c : s;
+  return (q & 2) ? -v : v;
+}
+
+// cos and sin of one angle, sharing the reduction.
+__device__ __forceinline__ void hm_sincos(float x, float &sn, float &cs)
+{
+  int   q;
+  float r = reduce_pio2(x, q);
+  float s = sin_poly(r), c = cos_poly(r);
+  // sin: q=0 s, 1 c, 2 -s, 3 -c ; cos: q=0 c, 1 -s, 2 -c, 3 s
+  sn = (q & 1) ? c : s;
+  sn = (q & 2) ? -sn : sn;
+  cs = (q & 1) ? s : c;
+  cs = ((q + 1) & 2) ? -cs : cs;
+}
+
+// OpenCL fract(): min(x - floor(x), 0x1.fffffep-1f).
+__device__ __forceinline__ float fract_cl(float x)
+{
+  return fminf(x - floorf(x), 0x1.fffffep-1f);
+}
+
+// wang_hash (HighMap/src/gpu_opencl/kernels/_common_rand.cl:66-74).
+__host__ __device__ __forceinline__ uint32_t wang_hash(uint32_t s)
+{
+  s = (s ^ 61u) ^ (s >> 16);
+  s *= 9u;
+  s = s ^ (s >> 4);
+  s *= 0x27d4eb2du;
+  s = s ^ (s >> 15);
+  return s;
+}
+
+// One xorshift32 (13, 17, 5) step (_common_rand.cl:76-82).
+__host__ __device__ __forceinline__ uint32_t xorshift32(uint32_t x)
+{
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  return x;
+}
+
+// hash12f (_common_rand.cl:12-17); dot product as two products and a sum.
+__device__ __forceinline__ float hash12f(float px, float py, float fseed)
+{
+  float dp = px * 127.1f + py * 311.7f;
+  return fract_cl(hm_sin(dp + fseed) * 43758.5453123f);
+}
+
+// ----------------------------------------------------------------- thermal
+
+// helper_thermal_exchange (HighMap/src/gpu_opencl/kernels/thermal.cl:5-24).
+__device__ __forceinline__ float th_exchange(float self,
+                                             float other,
+                                             float dist,
+                                             float talus)
+{
+  float max_dif = dist * talus;
+  if (self > other)
+  {
+    if (self - other > max_dif) return -0.2f * ((self - other) - max_dif) / dist;
+    return 0.f;
+  }
+  if (other - self > max_dif) return 0.2f * ((other - self) - max_dif) / dist;
+  return 0.f;
+}
+
+// Interior thermal update (thermal.cl:41-57): neighbours in the reference's
+// order (-1,0),(0,1),(0,-1),(1,0),(-1,-1),(-1,1),(1,-1),(1,1); distance
+// 1.414 (the literal) on diagonals.  n[k] are the neighbour heights in that
+// order.
+__device__ __forceinline__ float th_interior(float val, const float n[8], float t)
+{
+  float amount = 0.f;
+  amount += th_exchange(val, n[0], 1.f, t);
+  amount += th_exchange(val, n[1], 1.f, t);
+  amount += th_exchange(val, n[2], 1.f, t);
+  amount += th_exchange(val, n[3], 1.f, t);
+  amount += th_exchange(val, n[4], 1.414f, t);
+  amount += th_exchange(val, n[5], 1.414f, t);
+  amount += th_exchange(val, n[6], 1.414f, t);
+  amount += th_exchange(val, n[7], 1.414f, t);
+  return val + amount;
+}
+
+// ------------------------------------------------------------------ vpipes
+
+// Constants of one vpipes launch, formed on the host in FP32 exactly as the
+// reference forms them.
+struct VpConst
+{
+  int   nx, ny;
+  float dt;       // time step
+  float dtg;      // dt * gravity (gravity 1 in the reference)
+  float c0, c1;   // flux diffusion: strength and 1 - 4*strength
+  int   diffuse;  // flux_diffusion
+  float dmin;     // 0.001 * water_height / dt (water pass)
+  float fnx;      // (float)nx (erosion pass gradient scale)
+  float dmax;     // water_height (erosion pass)
+  float kc, ke, kd, kexp, ratio;
+  float ce;       // 1 - dt * evap_rate
+  float rr;       // rain_per_iter (extension)
+  int   maintain; // maintain_water_volume
+};
+
+// hydraulic_vpipes_flow_pass (hydraulic_vpipes.cl:33-68) for one cell.
+// Inputs: own z, d1; neighbour z, d1 (left, right, top = j+1, bottom = j-1);
+// own fluxes.  Output fluxes in (l, r, t, b).
+__device__ __forceinline__ void vp_flow(float        zc,
+                                        float        dc,
+                                        float        zl,
+                                        float        dl,
+                                        float        zr,
+                                        float        dr,
+                                        float        zt,
+                                        float        dtp,
+                                        float        zb,
+                                        float        db,
+                                        float       &l,
+                                        float       &r,
+                                        float       &t,
+                                        float       &b,
+                                        const VpConst &k)
+{
+  float h0 = zc + dc;
+  float dhl = h0 - zl - dl;
+  float dhr = h0 - zr - dr;
+  float dht = h0 - zt - dtp;
+  float dhb = h0 - zb - db;
+  l = fmaxf(0.f, l + k.dtg * dhl);
+  r = fmaxf(0.f, r + k.dtg * dhr);
+  t = fmaxf(0.f, t + k.dtg * dht);
+  b = fmaxf(0.f, b + k.dtg * dhb);
+  if (k.diffuse)
+  {
+    float f_diff = k.c0 * (l + r + t + b);
+    l = k.c1 * l + f_diff;
+    r = k.c1 * r + f_diff;
+    t = k.c1 * t + f_diff;
+    b = k.c1 * b + f_diff;
+  }
+  float sum = l + r + t + b;
+  float kk = 0.f;
+  if (sum > 1e-5f) kk = dc / (sum * k.dt);
+  kk = fminf(fmaxf(kk, 0.f), 1.f);
+  l = l * kk;
+  r = r * kk;
+  t = t * kk;
+  b = b * kk;
+}
+
+// Net water change of the water pass (hydraulic_vpipes.cl:97-102).
+// frl = fr(i-1), ftb = ft(j-1), flr = fl(i+1), fbt = fb(j+1).
+__device__ __forceinline__ float vp_depth(float        d1,
+                                          float        frl,
+                                          float        ftb,
+                                          float        flr,
+                                          float        fbt,
+                                          float        fl,
+                                          float        fr,
+                                          float        ft,
+                                          float        fb,
+                                          const VpConst &k)
+{
+  float dv = k.dt * (frl + ftb + flr + fbt - fl - fr - ft - fb);
+  return fmaxf(0.f, d1 + dv);
+}
+
+// Velocities of the water pass (hydraulic_vpipes.cl:104-115), given the
+// new depth d2.
+__device__ __forceinline__ void vp_velocity(float        d2,
+                                            float        frl,
+                                            float        ftb,
+                                            float        flr,
+                                            float        fbt,
+                                            float        fl,
+                                            float        fr,
+                                            float        ft,
+                                            float        fb,
+                                            float       &u,
+                                            float       &v,
+                                            const VpConst &k)
+{
+  float un = 0.5f * (frl - fl + fr - flr);
+  float vn = 0.5f * (ftb - fb + ft - fbt);
+  float dmean = fmaxf(k.dmin, d2);
+  u = un / dmean;
+  v = vn / dmean;
+}
+
+__device__ __forceinline__ float hm_hypot(float x, float y)
+{
+  return sqrtf(x * x + y * y);
+}
+
+// hydraulic_vpipes_erosion_pass (hydraulic_vpipes.cl:149-183) for one cell.
+// zr/zl/zt/zb and sr/sl/st/sb are the neighbour values right (i+1), left
+// (i-1), top (j+1), bottom (j-1).
+__device__ __forceinline__ void vp_erosion(float        zc,
+                                           float        sc,
+                                           float        zr,
+                                           float        zl,
+                                           float        zt,
+                                           float        zb,
+                                           float        sr,
+                                           float        sl,
+                                           float        st,
+                                           float        sb,
+                                           float        d2,
+                                           float        u,
+                                           float        v,
+                                           float       &zo,
+                                           float       &so,
+                                           const VpConst &k)
+{
+  float dzx = 0.5f * (zr - zl + sr - sl);
+  float dzy = 0.5f * (zt - zb + st - sb);
+  float talus = hm_hypot(dzx, dzy);
+  float dzn = fmaxf(1e-3f, k.fnx * talus);
+  float salpha = fmaxf(0.01f, dzn / hm_hypot(1.f, dzn));
+  float speed = hm_hypot(u, v);
+  float depth = fminf(d2 / k.dmax, k.ratio);
+  float discharge = depth * speed;
+  float pw = k.kexp == 1.f ? discharge : powf(discharge, k.kexp);
+  float capa = k.kc * pw * salpha;
+  if (capa > sc)
+  {
+    float amount = k.ke * (capa - sc);
+    zo = zc - amount;
+    so = sc + amount;
+  }
+  else
+  {
+    float amount = k.kd * (sc - capa);
+    zo = zc + amount;
+    so = sc - amount;
+  }
+}
+
+// One axis of the linear mirrored-repeat sampler (OpenCL 1.2 rule for
+// CLK_ADDRESS_MIRRORED_REPEAT | CLK_FILTER_LINEAR applied to p/w).
+__device__ __forceinline__ void mirror_axis(float p, int w, int &i0, int &i1, float &a)
+{
+  float fw = (float)w;
+  float sg = p / fw;
+  float sp = fabsf(sg - 2.f * __builtin_rintf(0.5f * sg));
+  float q = sp * fw - 0.5f;
+  float q0 = floorf(q);
+  a = q - q0;
+  int kq = (int)q0;
+  i0 = min(max(kq, 0), w - 1);
+  i1 = min(max(kq + 1, 0), w - 1);
+}
+
+// Sediment transport (hydraulic_vpipes.cl:204-210): bilinear sample of the
+// sediment plane at the backtrace (i - dt*u, j - dt*v).
+template <class S>
+__device__ __forceinline__ float vp_transport(const typename S::T *sp,
+                                              int                  i,
+                                              int                  j,
+                                              float                u,
+                                              float                v,
+                                              const VpConst       &k)
+{
+  float px = (float)i - k.dt * u;
+  float py = (float)j - k.dt * v;
+  int   i0, i1, j0, j1;
+  float a, b;
+  mirror_axis(px, k.nx, i0, i1, a);
+  mirror_axis(py, k.ny, j0, j1, b);
+  float  w00 = (1.f - a) * (1.f - b);
+  float  w10 = a * (1.f - b);
+  float  w01 = (1.f - a) * b;
+  float  w11 = a * b;
+  size_t r0 = (size_t)j0 * k.nx, r1 = (size_t)j1 * k.nx;
+  float  s00 = S::ld(sp[r0 + i0]), s10 = S::ld(sp[r0 + i1]);
+  float  s01 = S::ld(sp[r1 + i0]), s11 = S::ld(sp[r1 + i1]);
+  return w00 * s00 + w10 * s10 + w01 * s01 + w11 * s11;
+}
+
+// Water depth at the start of an iteration from the stored pre-evaporation
+// depth d2 of the previous one: evaporation, volume top-up and the optional
+// per-iteration rain (hydraulic_vpipes_gpu.cpp:163-169).
+__device__ __forceinline__ float vp_d1(float        d2prev,
+                                       float        rain,
+                                       int          has_prev,
+                                       int          topup,
+                                       float        wta,
+                                       float        rf,
+                                       const VpConst &k)
+{
+  float d = d2prev;
+  if (has_prev)
+  {
+    d = d2prev * k.ce;
+    if (topup) d = d + (wta * rain) / rf;
+  }
+  if (k.rr != 0.f) d = d + k.rr * rain;
+  return d;
+}
+
+// --------------------------------------------- exact fixed-point volume sum
+
+// A float x maps to the 128-bit integer round(x * 2^64) (exact for
+// |x| >= 2^-41); sums of such integers do not depend on order.
+struct Fix128
+{
+  uint64_t lo;
+  int64_t  hi;
+};
+
+__device__ __forceinline__ Fix128 fix_term(float x)
+{
+  uint32_t b = __float_as_uint(x);
+  uint32_t e = (b >> 23) & 0xffu;
+  uint32_t m = (b & 0x7fffffu) | 0x800000u;
+  uint64_t lo = 0, hi = 0;
+  if (e != 0)
+  {
+    int sh = (int)e - 86;
+    if (sh >= 0)
+    {
+      sh = sh > 103 ? 103 : sh;
+      if (sh < 64)
+      {
+        lo = (uint64_t)m << sh;
+        hi = sh > 40 ? ((uint64_t)m >> (64 - sh)) : 0;
+      }
+      else
+        hi = (uint64_t)m << (sh - 64);
+    }
+    else if (sh > -25)
+    {
+      uint32_t rsh = (uint32_t)(-sh);
+      uint32_t q = m >> rsh;
+      uint32_t rem = m & ((1u << rsh) - 1u);
+      uint32_t half = 1u << (rsh - 1);
+      if (rem > half || (rem == half && (q & 1u))) q++;
+      lo = q;
+    }
+  }
+  if (b >> 31)
+  {
+    lo = ~lo;
+    hi = ~hi;
+    lo += 1;
+    if (lo == 0) hi += 1;
+  }
+  return Fix128{lo, (int64_t)hi};
+}
+
+__device__ __forceinline__ void fix_add(Fix128 &a, const Fix128 &b)
+{
+  uint64_t lo = a.lo + b.lo;
+  uint64_t c = lo < a.lo ? 1u : 0u;
+  a.lo = lo;
+  a.hi = (int64_t)((uint64_t)a.hi + (uint64_t)b.hi + c);
+}
+
+__host__ __device__ __forceinline__ double fix_to_double(uint64_t lo, int64_t hi)
+{
+  double hid = (double)(int32_t)(hi >> 32) * 4294967296.0 +
+               (double)(uint32_t)(uint64_t)hi;
+  double lod = (double)(uint32_t)(lo >> 32) * 4294967296.0 + (double)(uint32_t)lo;
+  return hid + lod * 0x1p-64;
+}
+
+} // namespace hmg

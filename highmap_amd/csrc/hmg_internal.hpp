@@ -1,0 +1,152 @@
+// hmg_internal.hpp -- runtime internals of libhmg shared by its sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "hmg.h"
+#include "hmg_cell.hpp"
+
+namespace hmg
+{
+
+// Thread-local error message; returns `code` so call sites can
+// `return set_error(...)`.
+int set_error(int code, const char *fmt, ...);
+void clear_error();
+
+#define HMG_HIP(expr)                                                          \
+  do                                                                           \
+  {                                                                            \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::hmg::set_error(e_ == hipErrorOutOfMemory ? HMG_ERR_OOM          \
+                                                        : HMG_ERR_DEVICE,      \
+                              "%s failed: %s (%s:%d)",                         \
+                              #expr,                                           \
+                              hipGetErrorString(e_),                           \
+                              __FILE__,                                        \
+                              __LINE__);                                       \
+  } while (0)
+
+#define HMG_TRY(expr)                                                          \
+  do                                                                           \
+  {                                                                            \
+    int rc_ = (expr);                                                          \
+    if (rc_ != HMG_OK) return rc_;                                             \
+  } while (0)
+
+// Caching device allocator of one context: freed blocks are kept by size and
+// handed out again, so repeated drop-in calls do not hit hipMalloc.
+class Pool
+{
+public:
+  explicit Pool(int device) : device_(device) {}
+  ~Pool();
+  int  alloc(void **p, size_t bytes);
+  void release(void *p);
+  void trim();
+  size_t held_bytes() const { return held_; }
+
+private:
+  int                                  device_;
+  std::mutex                           mu_;
+  std::multimap<size_t, void *>        free_;
+  std::map<void *, size_t>             live_;
+  size_t                               held_ = 0;
+};
+
+// RAII device buffer from a Pool.
+struct DevBuf
+{
+  Pool  *pool = nullptr;
+  void  *ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  int get(Pool *pl, size_t n)
+  {
+    reset();
+    pool = pl;
+    bytes = n;
+    return pl->alloc(&ptr, n);
+  }
+  void reset()
+  {
+    if (ptr && pool) pool->release(ptr);
+    ptr = nullptr;
+  }
+  template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+} // namespace hmg
+
+struct hmg_ctx
+{
+  int             device = 0;
+  hipStream_t     stream = nullptr;
+  int             num_cus = 0;
+  int             wave_size = 64;
+  std::recursive_mutex mu;
+  hmg::Pool      *pool = nullptr;
+};
+
+namespace hmg
+{
+
+// Makes ctx->device current for the calling thread.
+int ctx_enter(hmg_ctx *ctx);
+
+// Checks the last kernel launch.
+inline int check_launch(const char *what)
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(HMG_ERR_DEVICE, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return HMG_OK;
+}
+
+// Builds the launch constants of vpipes from the parameters, in FP32 exactly
+// as the reference forms them (hydraulic_vpipes.cl:30-31,48-49,109,156-159;
+// hydraulic_vpipes_gpu.cpp:163).
+VpConst make_vp_const(const hmg_vpipes_params &p, int nx, int ny);
+
+int validate_vp_params(const hmg_vpipes_params &p);
+
+// Kernel-side entry points implemented in the .hip sources.
+int launch_thermal(hmg_ctx     *ctx,
+                   float       *z,
+                   float       *zscratch,
+                   int          nx,
+                   int          ny,
+                   const float *talus,
+                   float        talus_scalar,
+                   int          iterations,
+                   const float *bedrock,
+                   hipStream_t  st);
+int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
+int launch_noise_fbm(float       *out,
+                     int          nx,
+                     int          ny,
+                     float        kx,
+                     float        ky,
+                     uint32_t     seed,
+                     int          octaves,
+                     float        weight,
+                     float        persistence,
+                     float        lacunarity,
+                     const float *ctrl,
+                     const float *noise_x,
+                     const float *noise_y,
+                     const float *bbox4,
+                     const int   *period2,
+                     hipStream_t  st);
+
+} // namespace hmg

@@ -1,0 +1,962 @@
+// vpipes.hip -- hmap::gpu::hydraulic_vpipes for gfx950, and the
+// device-resident erosion session it runs on.
+//
+// Reference: host loop HighMap/src/erosion/hydraulic_vpipes_gpu.cpp:15-178,
+// kernels HighMap/src/gpu_opencl/kernels/hydraulic_vpipes.cl:5-211.
+//
+// State kept in HBM between launches (one session = one map):
+//   z    FP32            terrain height
+//   D    storage type    water depth after the last water pass, before
+//                        evaporation (d2 of the reference)
+//   F    4 x storage     fluxes fl, fr, ft, fb after the last flow pass
+//   Sp   storage type    sediment after the last erosion pass, before
+//                        transport
+// The reference iteration is flow -> water -> erosion -> transport ->
+// evaporation -> top-up.  Transport, evaporation and top-up only feed the
+// next iteration, so they are applied at the start of the next launch from
+// the stored state; u and v of the previous iteration are recomputed from F
+// and D exactly as the water pass formed them.  The result after k
+// iterations is identical, value for value, to the reference order.
+//
+// Water volume: the top-up needs sum(d) over the whole grid every
+// iteration.  Each launch adds round(d * 2^64) of its cells into 128-bit
+// integer slots (exact, order-independent), and the next launch turns the
+// slots into the top-up amount.  Three slot sets rotate: launch k reads set
+// k-1, adds into set k and clears set k+1.
+//
+// Kernels (storage type S in {f32, bf16, f16}):
+//   k_vp_fused     one launch per iteration.  A block owns a TW x TH tile,
+//                  stages z, water and fluxes of the tile plus a 2-cell halo
+//                  in LDS, then transport (1-cell halo), flow (1-cell halo),
+//                  water and erosion (tile), and writes z, D, F, Sp once.
+//                  HBM: 28 B read + 28 B written per cell (FP32 storage).
+//   k_vp_prep, k_vp_flow, k_vp_water, k_vp_erosion, k_vp_sum
+//                  the plain form: one launch per pass, every intermediate
+//                  through HBM, as the reference structures its loop.
+//   k_vp_export    forms d, s, u, v as hydraulic_vpipes returns them.
+#include <cstring>
+
+#include "hmg_internal.hpp"
+
+namespace hmg
+{
+
+constexpr int NSLOT = 16; // fixed-point accumulators per slot set
+
+struct VpDyn
+{
+  int    has_prev; // 0 on the first iteration of a session
+  int    topup;    // volume top-up pending from the previous iteration
+  double W0;       // initial water volume
+  float  rf;       // rain volume (float, as the reference divides by it)
+};
+
+__device__ __forceinline__ int clampi(int x, int n)
+{
+  return min(max(x, 0), n - 1);
+}
+
+// Top-up amount (W0 - sum(d)) of the previous iteration from its slots.
+__device__ __forceinline__ float slots_wta(const Fix128 *set, double W0)
+{
+  Fix128 acc{0, 0};
+  for (int q = 0; q < NSLOT; q++) fix_add(acc, set[q]);
+  return (float)(W0 - fix_to_double(acc.lo, (int64_t)acc.hi));
+}
+
+// Block-wide reduction of a Fix128 partial, then two atomics into one slot.
+// `red` is LDS of at least (blockDim.x / 64) entries.
+__device__ __forceinline__ void block_fix_atomic(Fix128 acc, Fix128 *red, Fix128 *set)
+{
+  for (int o = 32; o > 0; o >>= 1)
+  {
+    Fix128 other;
+    other.lo = __shfl_xor(acc.lo, o, 64);
+    other.hi = __shfl_xor(acc.hi, o, 64);
+    fix_add(acc, other);
+  }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    for (int q = 1; q < nw; q++) fix_add(acc, red[q]);
+    Fix128             *slot = set + (blockIdx.x % NSLOT);
+    unsigned long long  old = atomicAdd((unsigned long long *)&slot->lo, (unsigned long long)acc.lo);
+    unsigned long long  carry = (old + acc.lo < old) ? 1ull : 0ull;
+    atomicAdd((unsigned long long *)&slot->hi, (unsigned long long)acc.hi + carry);
+  }
+}
+
+__device__ __forceinline__ void clear_slots(Fix128 *set)
+{
+  if (blockIdx.x == 0 && threadIdx.x < NSLOT) set[threadIdx.x] = Fix128{0, 0};
+}
+
+// u, v of the previous iteration at cell (i, j) from its stored fluxes and
+// depth (the water pass, hydraulic_vpipes.cl:104-115).
+template <class S>
+__device__ __forceinline__ void prev_velocity(const Flux4<S> *__restrict__ F,
+                                              const typename S::T *__restrict__ D,
+                                              int i,
+                                              int j,
+                                              const VpConst &k,
+                                              float &u,
+                                              float &v)
+{
+  const int      nx = k.nx, ny = k.ny;
+  size_t         c = (size_t)j * nx + i;
+  Flux4<S>       fc = F[c];
+  float          frl = S::ld(F[(size_t)j * nx + clampi(i - 1, nx)].r);
+  float          flr = S::ld(F[(size_t)j * nx + clampi(i + 1, nx)].l);
+  float          ftb = S::ld(F[(size_t)clampi(j - 1, ny) * nx + i].t);
+  float          fbt = S::ld(F[(size_t)clampi(j + 1, ny) * nx + i].b);
+  vp_velocity(S::ld(D[c]), frl, ftb, flr, fbt, S::ld(fc.l), S::ld(fc.r), S::ld(fc.t),
+              S::ld(fc.b), u, v, k);
+}
+
+// ------------------------------------------------------------ plain form
+
+// Start of an iteration: transport of the stored sediment with the previous
+// velocities (or a copy on the first iteration), and the water depth after
+// evaporation, top-up and rain.
+template <class S>
+__global__ void __launch_bounds__(256) k_vp_prep(const Flux4<S> *__restrict__ F,
+                                                 const typename S::T *__restrict__ D,
+                                                 const typename S::T *__restrict__ Sp,
+                                                 const float *__restrict__ rain,
+                                                 float *__restrict__ d1,
+                                                 float *__restrict__ s,
+                                                 const Fix128 *__restrict__ prev_set,
+                                                 VpConst k,
+                                                 VpDyn   y)
+{
+  __shared__ float s_wta;
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    s_wta = (y.has_prev && y.topup) ? slots_wta(prev_set, y.W0) : 0.f;
+  __syncthreads();
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= k.nx || j >= k.ny) return;
+  size_t c = (size_t)j * k.nx + i;
+  float  r = rain ? rain[c] : 1.f;
+  d1[c] = vp_d1(S::ld(D[c]), r, y.has_prev, y.topup, s_wta, y.rf, k);
+  if (y.has_prev)
+  {
+    float u, v;
+    prev_velocity<S>(F, D, i, j, k, u, v);
+    s[c] = vp_transport<S>(Sp, i, j, u, v, k);
+  }
+  else
+    s[c] = S::ld(Sp[c]);
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_vp_flow(const float *__restrict__ z,
+                                                 const float *__restrict__ d1,
+                                                 const Flux4<S> *__restrict__ F,
+                                                 Flux4<S> *__restrict__ Fo,
+                                                 VpConst k)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= k.nx || j >= k.ny) return;
+  const int nx = k.nx, ny = k.ny;
+  size_t    c = (size_t)j * nx + i;
+  size_t    cl = (size_t)j * nx + clampi(i - 1, nx), cr = (size_t)j * nx + clampi(i + 1, nx);
+  size_t    ct = (size_t)clampi(j + 1, ny) * nx + i, cb = (size_t)clampi(j - 1, ny) * nx + i;
+  Flux4<S>  f = F[c];
+  float     l = S::ld(f.l), r = S::ld(f.r), t = S::ld(f.t), b = S::ld(f.b);
+  vp_flow(z[c], d1[c], z[cl], d1[cl], z[cr], d1[cr], z[ct], d1[ct], z[cb], d1[cb], l, r, t, b, k);
+  Flux4<S> o;
+  o.l = S::st(l);
+  o.r = S::st(r);
+  o.t = S::st(t);
+  o.b = S::st(b);
+  Fo[c] = o;
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_vp_water(const Flux4<S> *__restrict__ F,
+                                                  const float *__restrict__ d1,
+                                                  typename S::T *__restrict__ Do,
+                                                  float *__restrict__ uo,
+                                                  float *__restrict__ vo,
+                                                  VpConst k)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= k.nx || j >= k.ny) return;
+  const int nx = k.nx, ny = k.ny;
+  size_t    c = (size_t)j * nx + i;
+  Flux4<S>  fc = F[c];
+  float     frl = S::ld(F[(size_t)j * nx + clampi(i - 1, nx)].r);
+  float     flr = S::ld(F[(size_t)j * nx + clampi(i + 1, nx)].l);
+  float     ftb = S::ld(F[(size_t)clampi(j - 1, ny) * nx + i].t);
+  float     fbt = S::ld(F[(size_t)clampi(j + 1, ny) * nx + i].b);
+  float     fl = S::ld(fc.l), fr = S::ld(fc.r), ft = S::ld(fc.t), fb = S::ld(fc.b);
+  float     d2 = rnd<S>(vp_depth(d1[c], frl, ftb, flr, fbt, fl, fr, ft, fb, k));
+  float     u, v;
+  vp_velocity(d2, frl, ftb, flr, fbt, fl, fr, ft, fb, u, v, k);
+  Do[c] = S::st(d2);
+  uo[c] = u;
+  vo[c] = v;
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_vp_erosion(const float *__restrict__ z,
+                                                    const float *__restrict__ s,
+                                                    const typename S::T *__restrict__ D,
+                                                    const float *__restrict__ u,
+                                                    const float *__restrict__ v,
+                                                    float *__restrict__ zo,
+                                                    typename S::T *__restrict__ So,
+                                                    VpConst k)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= k.nx || j >= k.ny) return;
+  const int nx = k.nx, ny = k.ny;
+  size_t    c = (size_t)j * nx + i;
+  size_t    cl = (size_t)j * nx + clampi(i - 1, nx), cr = (size_t)j * nx + clampi(i + 1, nx);
+  size_t    ct = (size_t)clampi(j + 1, ny) * nx + i, cb = (size_t)clampi(j - 1, ny) * nx + i;
+  float     zn, sn;
+  vp_erosion(z[c], s[c], z[cr], z[cl], z[ct], z[cb], s[cr], s[cl], s[ct], s[cb], S::ld(D[c]),
+             u[c], v[c], zn, sn, k);
+  zo[c] = zn;
+  So[c] = S::st(sn);
+}
+
+// Sum of round(ld(a) * scale * 2^64) into a slot set.
+template <class S>
+__global__ void __launch_bounds__(256) k_vp_sum(const typename S::T *__restrict__ a,
+                                                size_t n,
+                                                float  scale,
+                                                int    use_scale,
+                                                Fix128 *set,
+                                                Fix128 *clear_set)
+{
+  __shared__ Fix128 red[4];
+  if (clear_set) clear_slots(clear_set);
+  Fix128 acc{0, 0};
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+  {
+    float x = S::ld(a[q]);
+    if (use_scale) x = x * scale;
+    fix_add(acc, fix_term(x));
+  }
+  block_fix_atomic(acc, red, set);
+}
+
+// ------------------------------------------------------------ fused form
+
+template <class S, int TW, int TH>
+struct FusedLds
+{
+  static constexpr int W2 = TW + 4, H2 = TH + 4; // tile + 2-cell halo
+  static constexpr int W1 = TW + 2, H1 = TH + 2; // tile + 1-cell halo
+  float z[H2][W2];
+  float d1[H2][W2];
+  float draw[H2][W2]; // stored depth of the previous iteration
+  float fl[H2][W2], fr[H2][W2], ft[H2][W2], fb[H2][W2];
+  float s[H1][W1];    // sediment after transport
+  Fix128 red[4];
+  float  wta;
+};
+
+template <class S, int TW, int TH>
+__global__ void __launch_bounds__(256) k_vp_fused(const float *__restrict__ z,
+                                                  const typename S::T *__restrict__ D,
+                                                  const Flux4<S> *__restrict__ F,
+                                                  const typename S::T *__restrict__ Sp,
+                                                  const float *__restrict__ rain,
+                                                  float *__restrict__ zo,
+                                                  typename S::T *__restrict__ Do,
+                                                  Flux4<S> *__restrict__ Fo,
+                                                  typename S::T *__restrict__ So,
+                                                  const Fix128 *__restrict__ prev_set,
+                                                  Fix128 *set,
+                                                  Fix128 *clear_set,
+                                                  int     tiles_x,
+                                                  int     n_tiles,
+                                                  VpConst k,
+                                                  VpDyn   y)
+{
+  using L = FusedLds<S, TW, TH>;
+  __shared__ L sm;
+  const int    nx = k.nx, ny = k.ny;
+  const int    tid = threadIdx.x;
+
+  if (tid == 0) sm.wta = (y.has_prev && y.topup) ? slots_wta(prev_set, y.W0) : 0.f;
+  if (k.maintain) clear_slots(clear_set);
+
+  // XCD-aware order: consecutive workgroups go to different XCDs, so each
+  // XCD is given a contiguous run of tiles (shared halos stay in its L2).
+  const int G = gridDim.x;
+  const int xcd = blockIdx.x & 7, lane_blk = blockIdx.x >> 3;
+  const int per_xcd = G >> 3;
+  const int lblk = (G & 7) ? blockIdx.x : xcd * per_xcd + lane_blk;
+
+  Fix128 acc{0, 0};
+  __syncthreads();
+  const float wta = sm.wta;
+
+  for (int tile = lblk; tile < n_tiles; tile += G)
+  {
+    const int ti0 = (tile % tiles_x) * TW, tj0 = (tile / tiles_x) * TH;
+
+    // Phase A: stage z, water and fluxes of tile + 2 (clamped to the edge).
+    for (int e = tid; e < L::W2 * L::H2; e += blockDim.x)
+    {
+      int      lx = e % L::W2, ly = e / L::W2;
+      int      gi = clampi(ti0 + lx - 2, nx), gj = clampi(tj0 + ly - 2, ny);
+      size_t   c = (size_t)gj * nx + gi;
+      float    dr = S::ld(D[c]);
+      Flux4<S> f = F[c];
+      float    r = rain ? rain[c] : 1.f;
+      sm.z[ly][lx] = z[c];
+      sm.draw[ly][lx] = dr;
+      sm.d1[ly][lx] = vp_d1(dr, r, y.has_prev, y.topup, wta, y.rf, k);
+      sm.fl[ly][lx] = S::ld(f.l);
+      sm.fr[ly][lx] = S::ld(f.r);
+      sm.ft[ly][lx] = S::ld(f.t);
+      sm.fb[ly][lx] = S::ld(f.b);
+    }
+    __syncthreads();
+
+    // Phase B: sediment transport on tile + 1 (reads the stored sediment
+    // plane at the backtrace position, anywhere in the grid).
+    for (int e = tid; e < L::W1 * L::H1; e += blockDim.x)
+    {
+      int lx = e % L::W1, ly = e / L::W1;
+      int gi = clampi(ti0 + lx - 1, nx), gj = clampi(tj0 + ly - 1, ny);
+      float sv;
+      if (y.has_prev)
+      {
+        // neighbours in the staged planes; indices clamped like the grid
+        int   cx = gi - ti0 + 2, cy = gj - tj0 + 2;
+        int   xl = clampi(gi - 1, nx) - ti0 + 2, xr = clampi(gi + 1, nx) - ti0 + 2;
+        int   yb = clampi(gj - 1, ny) - tj0 + 2, yt = clampi(gj + 1, ny) - tj0 + 2;
+        float u, v;
+        vp_velocity(sm.draw[cy][cx], sm.fr[cy][xl], sm.ft[yb][cx], sm.fl[cy][xr],
+                    sm.fb[yt][cx], sm.fl[cy][cx], sm.fr[cy][cx], sm.ft[cy][cx],
+                    sm.fb[cy][cx], u, v, k);
+        sv = vp_transport<S>(Sp, gi, gj, u, v, k);
+      }
+      else
+        sv = S::ld(Sp[(size_t)gj * nx + gi]);
+      sm.s[ly][lx] = sv;
+    }
+
+    // Phase C: flow on tile + 1 from the staged z, d1 and own fluxes.
+    constexpr int NF = (L::W1 * L::H1 + 255) / 256; // cells per lane
+    float         fo[NF][4];
+    int   nflow = 0;
+    for (int e = tid; e < L::W1 * L::H1; e += blockDim.x, nflow++)
+    {
+      int lx = e % L::W1 + 1, ly = e / L::W1 + 1; // staged-plane coordinates
+      int gi = clampi(ti0 + lx - 2, nx), gj = clampi(tj0 + ly - 2, ny);
+      int cx = gi - ti0 + 2, cy = gj - tj0 + 2;
+      int xl = clampi(gi - 1, nx) - ti0 + 2, xr = clampi(gi + 1, nx) - ti0 + 2;
+      int yb = clampi(gj - 1, ny) - tj0 + 2, yt = clampi(gj + 1, ny) - tj0 + 2;
+      float l = sm.fl[cy][cx], r = sm.fr[cy][cx], t = sm.ft[cy][cx], b = sm.fb[cy][cx];
+      vp_flow(sm.z[cy][cx], sm.d1[cy][cx], sm.z[cy][xl], sm.d1[cy][xl], sm.z[cy][xr],
+              sm.d1[cy][xr], sm.z[yt][cx], sm.d1[yt][cx], sm.z[yb][cx], sm.d1[yb][cx], l, r,
+              t, b, k);
+      fo[nflow][0] = rnd<S>(l);
+      fo[nflow][1] = rnd<S>(r);
+      fo[nflow][2] = rnd<S>(t);
+      fo[nflow][3] = rnd<S>(b);
+    }
+    __syncthreads();
+    nflow = 0;
+    for (int e = tid; e < L::W1 * L::H1; e += blockDim.x, nflow++)
+    {
+      int lx = e % L::W1 + 1, ly = e / L::W1 + 1;
+      sm.fl[ly][lx] = fo[nflow][0];
+      sm.fr[ly][lx] = fo[nflow][1];
+      sm.ft[ly][lx] = fo[nflow][2];
+      sm.fb[ly][lx] = fo[nflow][3];
+    }
+    __syncthreads();
+
+    // Phase D: water and erosion on the tile; write the new state.
+    for (int e = tid; e < TW * TH; e += blockDim.x)
+    {
+      int lx = e % TW, ly = e / TW;
+      int gi = ti0 + lx, gj = tj0 + ly;
+      if (gi >= nx || gj >= ny) continue;
+      int   cx = lx + 2, cy = ly + 2;
+      int   xl = clampi(gi - 1, nx) - ti0 + 2, xr = clampi(gi + 1, nx) - ti0 + 2;
+      int   yb = clampi(gj - 1, ny) - tj0 + 2, yt = clampi(gj + 1, ny) - tj0 + 2;
+      float fl = sm.fl[cy][cx], fr = sm.fr[cy][cx], ft = sm.ft[cy][cx], fb = sm.fb[cy][cx];
+      float frl = sm.fr[cy][xl], flr = sm.fl[cy][xr], ftb = sm.ft[yb][cx], fbt = sm.fb[yt][cx];
+      float d2 = rnd<S>(vp_depth(sm.d1[cy][cx], frl, ftb, flr, fbt, fl, fr, ft, fb, k));
+      float u, v;
+      vp_velocity(d2, frl, ftb, flr, fbt, fl, fr, ft, fb, u, v, k);
+      // sediment plane is staged on tile + 1: offset 1
+      float zn, sn;
+      vp_erosion(sm.z[cy][cx], sm.s[cy - 1][cx - 1], sm.z[cy][xr], sm.z[cy][xl],
+                 sm.z[yt][cx], sm.z[yb][cx], sm.s[cy - 1][xr - 1], sm.s[cy - 1][xl - 1],
+                 sm.s[yt - 1][cx - 1], sm.s[yb - 1][cx - 1], d2, u, v, zn, sn, k);
+      size_t c = (size_t)gj * nx + gi;
+      zo[c] = zn;
+      So[c] = S::st(sn);
+      Do[c] = S::st(d2);
+      Flux4<S> o;
+      o.l = S::st(fl);
+      o.r = S::st(fr);
+      o.t = S::st(ft);
+      o.b = S::st(fb);
+      Fo[c] = o;
+      if (k.maintain) fix_add(acc, fix_term(d2 * k.ce));
+    }
+    __syncthreads();
+  }
+  if (k.maintain) block_fix_atomic(acc, sm.red, set);
+}
+
+// ---------------------------------------------------------------- export
+
+template <class S>
+__global__ void __launch_bounds__(256) k_vp_export(const Flux4<S> *__restrict__ F,
+                                                   const typename S::T *__restrict__ D,
+                                                   const typename S::T *__restrict__ Sp,
+                                                   const float *__restrict__ rain,
+                                                   float *__restrict__ d_out,
+                                                   float *__restrict__ s_out,
+                                                   float *__restrict__ u_out,
+                                                   float *__restrict__ v_out,
+                                                   const Fix128 *__restrict__ prev_set,
+                                                   VpConst k,
+                                                   VpDyn   y)
+{
+  __shared__ float s_wta;
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    s_wta = (y.has_prev && y.topup) ? slots_wta(prev_set, y.W0) : 0.f;
+  __syncthreads();
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= k.nx || j >= k.ny) return;
+  size_t c = (size_t)j * k.nx + i;
+  float  dr = S::ld(D[c]);
+  if (!y.has_prev)
+  {
+    d_out[c] = dr;
+    s_out[c] = S::ld(Sp[c]);
+    u_out[c] = 0.f;
+    v_out[c] = 0.f;
+    return;
+  }
+  VpConst k0 = k;
+  k0.rr = 0.f; // state d excludes the rain of the next iteration
+  d_out[c] = vp_d1(dr, rain ? rain[c] : 1.f, 1, y.topup, s_wta, y.rf, k0);
+  float u, v;
+  prev_velocity<S>(F, D, i, j, k, u, v);
+  s_out[c] = vp_transport<S>(Sp, i, j, u, v, k);
+  u_out[c] = u;
+  v_out[c] = v;
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_fill_state(typename S::T *__restrict__ D,
+                                                    typename S::T *__restrict__ Sp,
+                                                    Flux4<S> *__restrict__ F,
+                                                    const float *__restrict__ rain,
+                                                    float wh,
+                                                    size_t n)
+{
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+  {
+    // hydraulic_vpipes_gpu.cpp:39,53: d = water_height * rain
+    D[q] = S::st(wh * (rain ? rain[q] : 1.f));
+    Sp[q] = S::st(0.f);
+    Flux4<S> f;
+    f.l = f.r = f.t = f.b = S::st(0.f);
+    F[q] = f;
+  }
+}
+
+__global__ void k_zero_fix(Fix128 *p, int n)
+{
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = Fix128{0, 0};
+}
+
+} // namespace hmg
+
+// ---------------------------------------------------------------- session
+
+using namespace hmg;
+
+struct hmg_session
+{
+  hmg_ctx          *ctx = nullptr;
+  int               nx = 0, ny = 0;
+  int               storage = HMG_STORAGE_F32;
+  hmg_vpipes_params p{};
+  VpConst           k{};
+  size_t            esize = 4; // bytes per stored scalar
+  DevBuf            z[2], D[2], Sp[2], F[2], rain, slots;
+  DevBuf            d1, s, u, v; // plain-form intermediates
+  DevBuf            zscratch;    // thermal ping-pong
+  int               cur = 0;
+  long              iter = 0;
+  double            W0 = 0.0, R = 0.0;
+  float             rf = 0.f;
+  int               tile_w = 64, tile_h = 16;
+};
+
+namespace hmg
+{
+
+static VpDyn make_dyn(const hmg_session *s)
+{
+  VpDyn y;
+  y.has_prev = s->iter > 0;
+  y.topup = s->p.maintain_water_volume ? 1 : 0;
+  y.W0 = s->W0;
+  y.rf = s->rf;
+  return y;
+}
+
+static Fix128 *slot_set(hmg_session *s, long it)
+{
+  return s->slots.as<Fix128>() + (size_t)(((it % 3) + 3) % 3) * NSLOT;
+}
+
+template <class S>
+static int session_init_state(hmg_session *s)
+{
+  size_t      n = (size_t)s->nx * s->ny;
+  hipStream_t st = s->ctx->stream;
+  int         nb = (int)((n + 255) / 256);
+  if (nb > 4096) nb = 4096;
+  k_fill_state<S><<<nb, 256, 0, st>>>((typename S::T *)s->D[0].ptr,
+                                      (typename S::T *)s->Sp[0].ptr,
+                                      (Flux4<S> *)s->F[0].ptr,
+                                      s->rain.as<float>(),
+                                      s->p.water_height,
+                                      n);
+  HMG_TRY(check_launch("fill_state"));
+  k_zero_fix<<<1, 64, 0, st>>>(s->slots.as<Fix128>(), 4 * NSLOT);
+  HMG_TRY(check_launch("zero_fix"));
+  // W0 = sum(d) and R = sum(rain) (hydraulic_vpipes_gpu.cpp:54-55), exact
+  Fix128 *w_set = s->slots.as<Fix128>() + 3 * NSLOT;
+  k_vp_sum<S><<<nb, 256, 0, st>>>((const typename S::T *)s->D[0].ptr, n, 1.f, 0, w_set,
+                                  nullptr);
+  HMG_TRY(check_launch("vp_sum(W0)"));
+  Fix128 h[NSLOT];
+  HMG_HIP(hipMemcpyAsync(h, w_set, sizeof(h), hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  Fix128 acc{0, 0};
+  for (int q = 0; q < NSLOT; q++)
+  {
+    uint64_t lo = acc.lo + h[q].lo;
+    uint64_t c = lo < acc.lo ? 1 : 0;
+    acc.lo = lo;
+    acc.hi = (int64_t)((uint64_t)acc.hi + (uint64_t)h[q].hi + c);
+  }
+  s->W0 = fix_to_double(acc.lo, acc.hi);
+  if (s->rain.ptr)
+  {
+    k_zero_fix<<<1, 64, 0, st>>>(w_set, NSLOT);
+    HMG_TRY(check_launch("zero_fix"));
+    k_vp_sum<StF32><<<nb, 256, 0, st>>>(s->rain.as<float>(), n, 1.f, 0, w_set, nullptr);
+    HMG_TRY(check_launch("vp_sum(R)"));
+    HMG_HIP(hipMemcpyAsync(h, w_set, sizeof(h), hipMemcpyDeviceToHost, st));
+    HMG_HIP(hipStreamSynchronize(st));
+    acc = Fix128{0, 0};
+    for (int q = 0; q < NSLOT; q++)
+    {
+      uint64_t lo = acc.lo + h[q].lo;
+      uint64_t c = lo < acc.lo ? 1 : 0;
+      acc.lo = lo;
+      acc.hi = (int64_t)((uint64_t)acc.hi + (uint64_t)h[q].hi + c);
+    }
+    s->R = fix_to_double(acc.lo, acc.hi);
+  }
+  else
+    s->R = (double)n;
+  s->rf = (float)s->R;
+  return HMG_OK;
+}
+
+template <class S>
+static int session_step_plain(hmg_session *s)
+{
+  using T = typename S::T;
+  const int   nx = s->nx, ny = s->ny;
+  hipStream_t st = s->ctx->stream;
+  dim3        blk(64, 4), grd((nx + 63) / 64, (ny + 3) / 4);
+  const int   a = s->cur, b = s->cur ^ 1;
+  VpDyn       y = make_dyn(s);
+  k_vp_prep<S><<<grd, blk, 0, st>>>((const Flux4<S> *)s->F[a].ptr, (const T *)s->D[a].ptr,
+                                    (const T *)s->Sp[a].ptr, s->rain.as<float>(),
+                                    s->d1.as<float>(), s->s.as<float>(),
+                                    slot_set(s, s->iter - 1), s->k, y);
+  HMG_TRY(check_launch("vp_prep"));
+  k_vp_flow<S><<<grd, blk, 0, st>>>(s->z[a].as<float>(), s->d1.as<float>(),
+                                    (const Flux4<S> *)s->F[a].ptr, (Flux4<S> *)s->F[b].ptr,
+                                    s->k);
+  HMG_TRY(check_launch("vp_flow"));
+  k_vp_water<S><<<grd, blk, 0, st>>>((const Flux4<S> *)s->F[b].ptr, s->d1.as<float>(),
+                                     (T *)s->D[b].ptr, s->u.as<float>(), s->v.as<float>(),
+                                     s->k);
+  HMG_TRY(check_launch("vp_water"));
+  k_vp_erosion<S><<<grd, blk, 0, st>>>(s->z[a].as<float>(), s->s.as<float>(),
+                                       (const T *)s->D[b].ptr, s->u.as<float>(),
+                                       s->v.as<float>(), s->z[b].as<float>(),
+                                       (T *)s->Sp[b].ptr, s->k);
+  HMG_TRY(check_launch("vp_erosion"));
+  if (s->p.maintain_water_volume)
+  {
+    size_t n = (size_t)nx * ny;
+    int    nb = (int)((n + 255) / 256);
+    if (nb > 2048) nb = 2048;
+    k_vp_sum<S><<<nb, 256, 0, st>>>((const T *)s->D[b].ptr, n, s->k.ce, 1,
+                                    slot_set(s, s->iter), slot_set(s, s->iter + 1));
+    HMG_TRY(check_launch("vp_sum"));
+  }
+  s->cur = b;
+  s->iter++;
+  return HMG_OK;
+}
+
+template <class S, int TW, int TH>
+static int session_step_fused(hmg_session *s)
+{
+  using T = typename S::T;
+  const int   nx = s->nx, ny = s->ny;
+  hipStream_t st = s->ctx->stream;
+  const int   a = s->cur, b = s->cur ^ 1;
+  const int   tiles_x = (nx + TW - 1) / TW, tiles_y = (ny + TH - 1) / TH;
+  const int   n_tiles = tiles_x * tiles_y;
+  int         G = s->ctx->num_cus * 4;
+  if (G > n_tiles) G = n_tiles;
+  VpDyn y = make_dyn(s);
+  k_vp_fused<S, TW, TH><<<G, 256, 0, st>>>(
+      s->z[a].as<float>(), (const T *)s->D[a].ptr, (const Flux4<S> *)s->F[a].ptr,
+      (const T *)s->Sp[a].ptr, s->rain.as<float>(), s->z[b].as<float>(), (T *)s->D[b].ptr,
+      (Flux4<S> *)s->F[b].ptr, (T *)s->Sp[b].ptr, slot_set(s, s->iter - 1),
+      slot_set(s, s->iter), slot_set(s, s->iter + 1), tiles_x, n_tiles, s->k, y);
+  HMG_TRY(check_launch("vp_fused"));
+  s->cur = b;
+  s->iter++;
+  return HMG_OK;
+}
+
+template <class S>
+static int session_step(hmg_session *s)
+{
+  if (s->p.mode == HMG_VP_PLAIN) return session_step_plain<S>(s);
+  return session_step_fused<S, 64, 16>(s);
+}
+
+template <class S>
+static int session_export(hmg_session *s, float *d, float *sd, float *u, float *v)
+{
+  using T = typename S::T;
+  const int   nx = s->nx, ny = s->ny;
+  hipStream_t st = s->ctx->stream;
+  dim3        blk(64, 4), grd((nx + 63) / 64, (ny + 3) / 4);
+  VpDyn       y = make_dyn(s);
+  k_vp_export<S><<<grd, blk, 0, st>>>((const Flux4<S> *)s->F[s->cur].ptr,
+                                      (const T *)s->D[s->cur].ptr,
+                                      (const T *)s->Sp[s->cur].ptr, s->rain.as<float>(), d,
+                                      sd, u, v, slot_set(s, s->iter - 1), s->k, y);
+  return check_launch("vp_export");
+}
+
+static int dispatch_step(hmg_session *s)
+{
+  switch (s->storage)
+  {
+  case HMG_STORAGE_BF16: return session_step<StBF16>(s);
+  case HMG_STORAGE_F16: return session_step<StF16>(s);
+  default: return session_step<StF32>(s);
+  }
+}
+
+static int session_create_common(hmg_ctx                 *ctx,
+                                 const float             *z,
+                                 bool                     z_on_dev,
+                                 int                      nx,
+                                 int                      ny,
+                                 const hmg_vpipes_params *p,
+                                 const float             *rain,
+                                 bool                     rain_on_dev,
+                                 int                      storage,
+                                 hmg_session            **out)
+{
+  clear_error();
+  if (!out) return set_error(HMG_ERR_INVALID, "null output pointer");
+  *out = nullptr;
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!z) return set_error(HMG_ERR_INVALID, "null z");
+  if (nx < 1 || ny < 1) return set_error(HMG_ERR_INVALID, "shape must be positive");
+  if ((size_t)nx * (size_t)ny > ((size_t)1 << 31))
+    return set_error(HMG_ERR_INVALID, "grid larger than 2^31 cells");
+  if (storage != HMG_STORAGE_F32 && storage != HMG_STORAGE_BF16 && storage != HMG_STORAGE_F16)
+    return set_error(HMG_ERR_INVALID, "unknown storage %d", storage);
+  hmg_vpipes_params prm;
+  if (p)
+    prm = *p;
+  else
+    hmg_vpipes_defaults(&prm);
+  HMG_TRY(validate_vp_params(prm));
+
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  hmg_session *s = new hmg_session();
+  s->ctx = ctx;
+  s->nx = nx;
+  s->ny = ny;
+  s->storage = storage;
+  s->p = prm;
+  s->k = make_vp_const(prm, nx, ny);
+  s->esize = storage == HMG_STORAGE_F32 ? 4 : 2;
+  const size_t n = (size_t)nx * ny;
+  int          rc = HMG_OK;
+  for (int q = 0; q < 2 && rc == HMG_OK; q++)
+  {
+    rc = s->z[q].get(ctx->pool, n * 4);
+    if (rc == HMG_OK) rc = s->D[q].get(ctx->pool, n * s->esize);
+    if (rc == HMG_OK) rc = s->Sp[q].get(ctx->pool, n * s->esize);
+    if (rc == HMG_OK) rc = s->F[q].get(ctx->pool, n * s->esize * 4);
+  }
+  if (rc == HMG_OK) rc = s->slots.get(ctx->pool, sizeof(Fix128) * 4 * NSLOT);
+  if (rc == HMG_OK && rain) rc = s->rain.get(ctx->pool, n * 4);
+  if (rc == HMG_OK && prm.mode == HMG_VP_PLAIN)
+  {
+    rc = s->d1.get(ctx->pool, n * 4);
+    if (rc == HMG_OK) rc = s->s.get(ctx->pool, n * 4);
+    if (rc == HMG_OK) rc = s->u.get(ctx->pool, n * 4);
+    if (rc == HMG_OK) rc = s->v.get(ctx->pool, n * 4);
+  }
+  if (rc != HMG_OK)
+  {
+    delete s;
+    return rc;
+  }
+  hipStream_t st = ctx->stream;
+  hipError_t  e = hipMemcpyAsync(s->z[0].ptr, z, n * 4,
+                                 z_on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && rain)
+    e = hipMemcpyAsync(s->rain.ptr, rain, n * 4,
+                       rain_on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st);
+  if (e != hipSuccess)
+  {
+    delete s;
+    return set_error(HMG_ERR_DEVICE, "state upload failed: %s", hipGetErrorString(e));
+  }
+  switch (storage)
+  {
+  case HMG_STORAGE_BF16: rc = session_init_state<StBF16>(s); break;
+  case HMG_STORAGE_F16: rc = session_init_state<StF16>(s); break;
+  default: rc = session_init_state<StF32>(s); break;
+  }
+  if (rc != HMG_OK)
+  {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return HMG_OK;
+}
+
+} // namespace hmg
+
+extern "C" {
+
+size_t hmg_session_bytes(int nx, int ny, int storage)
+{
+  size_t n = (size_t)(nx > 0 ? nx : 0) * (size_t)(ny > 0 ? ny : 0);
+  size_t e = storage == HMG_STORAGE_F32 ? 4 : 2;
+  return 2 * n * (4 + e * 6) + sizeof(Fix128) * 4 * NSLOT;
+}
+
+int hmg_session_create(hmg_ctx                 *ctx,
+                       const float             *z_host,
+                       int                      nx,
+                       int                      ny,
+                       const hmg_vpipes_params *p,
+                       const float             *rain_host,
+                       int                      storage,
+                       hmg_session            **out)
+{
+  return session_create_common(ctx, z_host, false, nx, ny, p, rain_host, false, storage, out);
+}
+
+int hmg_session_create_dev(hmg_ctx                 *ctx,
+                           const float             *z_dev,
+                           int                      nx,
+                           int                      ny,
+                           const hmg_vpipes_params *p,
+                           const float             *rain_dev,
+                           int                      storage,
+                           hmg_session            **out)
+{
+  return session_create_common(ctx, z_dev, true, nx, ny, p, rain_dev, true, storage, out);
+}
+
+int hmg_session_vpipes(hmg_session *s, int iterations)
+{
+  clear_error();
+  if (!s) return set_error(HMG_ERR_INVALID, "null session");
+  if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HMG_TRY(ctx_enter(s->ctx));
+  for (int it = 0; it < iterations; it++) HMG_TRY(dispatch_step(s));
+  return HMG_OK;
+}
+
+int hmg_session_thermal(hmg_session *s, float talus, int iterations)
+{
+  clear_error();
+  if (!s) return set_error(HMG_ERR_INVALID, "null session");
+  if (s->nx < 3 || s->ny < 3) return set_error(HMG_ERR_INVALID, "thermal needs nx, ny >= 3");
+  if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HMG_TRY(ctx_enter(s->ctx));
+  float *zc = s->z[s->cur].as<float>();
+  float *zt = s->z[s->cur ^ 1].as<float>();
+  HMG_TRY(launch_thermal(s->ctx, zc, zt, s->nx, s->ny, nullptr, talus, iterations, nullptr,
+                         s->ctx->stream));
+  return launch_extrapolate_borders(zc, s->nx, s->ny, s->ctx->stream);
+}
+
+int hmg_session_export(hmg_session *s, float *z_host, float *d_host, float *s_host,
+                       float *u_host, float *v_host)
+{
+  clear_error();
+  if (!s) return set_error(HMG_ERR_INVALID, "null session");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HMG_TRY(ctx_enter(s->ctx));
+  const size_t n = (size_t)s->nx * s->ny;
+  hipStream_t  st = s->ctx->stream;
+  if (z_host)
+    HMG_HIP(hipMemcpyAsync(z_host, s->z[s->cur].ptr, n * 4, hipMemcpyDeviceToHost, st));
+  if (d_host || s_host || u_host || v_host)
+  {
+    DevBuf out;
+    HMG_TRY(out.get(s->ctx->pool, n * 16));
+    float *o = out.as<float>();
+    int    rc;
+    switch (s->storage)
+    {
+    case HMG_STORAGE_BF16: rc = session_export<StBF16>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+    case HMG_STORAGE_F16: rc = session_export<StF16>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+    default: rc = session_export<StF32>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+    }
+    HMG_TRY(rc);
+    float *dst[4] = {d_host, s_host, u_host, v_host};
+    for (int q = 0; q < 4; q++)
+      if (dst[q]) HMG_HIP(hipMemcpyAsync(dst[q], o + q * n, n * 4, hipMemcpyDeviceToHost, st));
+    HMG_HIP(hipStreamSynchronize(st));
+  }
+  HMG_HIP(hipStreamSynchronize(st));
+  return HMG_OK;
+}
+
+int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d)
+{
+  clear_error();
+  if (!s) return set_error(HMG_ERR_INVALID, "null session");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HMG_TRY(ctx_enter(s->ctx));
+  const size_t n = (size_t)s->nx * s->ny;
+  hipStream_t  st = s->ctx->stream;
+  DevBuf       out, acc;
+  HMG_TRY(out.get(s->ctx->pool, n * 16));
+  HMG_TRY(acc.get(s->ctx->pool, sizeof(Fix128) * 3 * NSLOT));
+  float *o = out.as<float>();
+  int    rc;
+  switch (s->storage)
+  {
+  case HMG_STORAGE_BF16: rc = session_export<StBF16>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+  case HMG_STORAGE_F16: rc = session_export<StF16>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+  default: rc = session_export<StF32>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+  }
+  HMG_TRY(rc);
+  Fix128 *sets = acc.as<Fix128>();
+  k_zero_fix<<<1, 64, 0, st>>>(sets, 3 * NSLOT);
+  HMG_TRY(check_launch("zero_fix"));
+  int nb = (int)((n + 255) / 256);
+  if (nb > 2048) nb = 2048;
+  const float *src[3] = {s->z[s->cur].as<float>(), o + n, o};
+  for (int q = 0; q < 3; q++)
+  {
+    k_vp_sum<StF32><<<nb, 256, 0, st>>>(src[q], n, 1.f, 0, sets + q * NSLOT, nullptr);
+    HMG_TRY(check_launch("vp_sum"));
+  }
+  Fix128 h[3 * NSLOT];
+  HMG_HIP(hipMemcpyAsync(h, sets, sizeof(h), hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  double *dst[3] = {sum_z, sum_s, sum_d};
+  for (int q = 0; q < 3; q++)
+  {
+    Fix128 a{0, 0};
+    for (int r = 0; r < NSLOT; r++)
+    {
+      const Fix128 &b = h[q * NSLOT + r];
+      uint64_t      lo = a.lo + b.lo;
+      uint64_t      c = lo < a.lo ? 1 : 0;
+      a.lo = lo;
+      a.hi = (int64_t)((uint64_t)a.hi + (uint64_t)b.hi + c);
+    }
+    if (dst[q]) *dst[q] = fix_to_double(a.lo, a.hi);
+  }
+  return HMG_OK;
+}
+
+float *hmg_session_z_dev(hmg_session *s)
+{
+  return s ? s->z[s->cur].as<float>() : nullptr;
+}
+
+int hmg_session_sync(hmg_session *s)
+{
+  if (!s) return set_error(HMG_ERR_INVALID, "null session");
+  return hmg_sync(s->ctx);
+}
+
+int hmg_session_destroy(hmg_session *s)
+{
+  if (!s) return HMG_OK;
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return HMG_OK;
+}
+
+int hmg_vpipes(hmg_ctx                 *ctx,
+               float                   *z_host,
+               int                      nx,
+               int                      ny,
+               const hmg_vpipes_params *p,
+               const float             *rain_host,
+               float                   *d_out_host,
+               float                   *s_out_host,
+               float                   *u_out_host,
+               float                   *v_out_host)
+{
+  hmg_vpipes_params prm;
+  if (p)
+    prm = *p;
+  else
+    hmg_vpipes_defaults(&prm);
+  hmg_session *s = nullptr;
+  HMG_TRY(hmg_session_create(ctx, z_host, nx, ny, &prm, rain_host, HMG_STORAGE_F32, &s));
+  int rc = hmg_session_vpipes(s, prm.iterations);
+  if (rc == HMG_OK) rc = hmg_session_export(s, z_host, d_out_host, s_out_host, u_out_host,
+                                            v_out_host);
+  hmg_session_destroy(s);
+  return rc;
+}
+
+} // extern "C"

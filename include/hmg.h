@@ -1,0 +1,268 @@
+/*
+ * hmg.h -- C ABI of libhmg, the MI355X (gfx950) implementation of HighMap's
+ * grid-erosion hot path: gpu::noise_fbm (PERLIN), gpu::thermal and
+ * gpu::hydraulic_vpipes.
+ *
+ * This header replaces the OpenCL runtime layer the reference binds through
+ * (HighMap/src/gpu_opencl/gpu_opencl.cpp and the CLWrapper `Run` calls in
+ * HighMap/src/erosion/hydraulic_vpipes_gpu.cpp:67-159,
+ * HighMap/src/erosion/thermal_gpu.cpp:26-67 and
+ * HighMap/src/primitives/coherent_noise/primitives_gpu.cpp:558-588).
+ * The C++ drop-in (include/hmap_gpu.hpp) keeps the reference's hmap::gpu::
+ * signatures on top of it.
+ *
+ * Conventions
+ *  - Arrays are float32, i-fastest, linear index j*nx + i (hmap::Array,
+ *    HighMap/include/highmap/array.hpp:236-239).
+ *  - Pointers named *_host are host memory, borrowed for the call; the call
+ *    is synchronous.  Pointers named *_dev are device memory on the context's
+ *    device; those calls are asynchronous on the context stream (or on the
+ *    stream given) and return once queued.
+ *  - Every entry point returns 0 (HMG_OK) or a negative hmg_status;
+ *    hmg_last_error() then holds a message for the calling thread.
+ *  - One context may be used from several threads; calls on it serialise.
+ *    Separate contexts on one device run concurrently on their own streams.
+ */
+#ifndef HMG_H
+#define HMG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HMG_ABI_VERSION 1
+
+typedef enum hmg_status
+{
+  HMG_OK = 0,
+  HMG_ERR_INVALID = -1,     /* bad argument (size, null, range)          */
+  HMG_ERR_OOM = -2,         /* device memory exhausted / budget exceeded */
+  HMG_ERR_DEVICE = -3,      /* HIP runtime or kernel error               */
+  HMG_ERR_NONFINITE = -4,   /* NaN/Inf detected in the state             */
+  HMG_ERR_UNSUPPORTED = -5, /* option outside the implemented path       */
+  HMG_ERR_NOT_READY = -6    /* asynchronous result not available yet     */
+} hmg_status;
+
+/* Storage type of the water, sediment and flux planes in a session.
+ * Arithmetic is FP32 in every case; z is always FP32. */
+typedef enum hmg_storage
+{
+  HMG_STORAGE_F32 = 0,
+  HMG_STORAGE_BF16 = 1,
+  HMG_STORAGE_F16 = 2
+} hmg_storage;
+
+/* How vpipes iterations are launched. */
+typedef enum hmg_vp_mode
+{
+  HMG_VP_FUSED = 0, /* one launch per iteration (default)                  */
+  HMG_VP_PLAIN = 1  /* the reference's pass structure: flow, water,
+                       erosion, transport, evaporation, top-up launches    */
+} hmg_vp_mode;
+
+/* Parameters of hydraulic_vpipes.  The first 13 fields and their defaults
+ * are those of hmap::gpu::hydraulic_vpipes
+ * (HighMap/include/highmap/erosion.hpp:1206-1223):
+ *   water_height 1e-2, maintain_water_volume 1, evap_rate 0.1,
+ *   iterations 50, dt 0.5, k_capacity 0.5, k_erode 1e-3, k_depose 1e-2,
+ *   k_discharge_exp 1, downcutting_max_depth_ratio 10, flux_diffusion 1,
+ *   flux_diffusion_strength 0.01.
+ * gravity (reference: hard-coded 1, hydraulic_vpipes.cl:31) and
+ * rain_per_iter (reference: none; water added before each flow pass as
+ * rain_per_iter * rain_map) are extensions.  Use hmg_vpipes_defaults(). */
+typedef struct hmg_vpipes_params
+{
+  float water_height;
+  float evap_rate;
+  float dt;
+  float k_capacity;
+  float k_erode;
+  float k_depose;
+  float k_discharge_exp;
+  float downcutting_max_depth_ratio;
+  float flux_diffusion_strength;
+  float gravity;
+  float rain_per_iter;
+  int   maintain_water_volume;
+  int   flux_diffusion;
+  int   iterations;
+  int   mode; /* hmg_vp_mode */
+} hmg_vpipes_params;
+
+typedef struct hmg_ctx     hmg_ctx;
+typedef struct hmg_session hmg_session;
+
+/* ------------------------------------------------------------ runtime -- */
+
+/* Creates a context on HIP device `device`.  Replaces
+ * hmap::gpu::init_opencl() (HighMap/src/gpu_opencl/gpu_opencl.cpp:32-134,
+ * decl. HighMap/include/highmap/opencl/gpu_opencl.hpp:30), which returns
+ * false when no device is ready; here that is HMG_ERR_DEVICE. */
+int hmg_init(int device, hmg_ctx **out);
+int hmg_destroy(hmg_ctx *ctx);
+/* Thread-local message of the last failing call ("" if none). */
+const char *hmg_last_error(void);
+int         hmg_abi_version(void);
+/* Fills defaults of hmap::gpu::hydraulic_vpipes plus gravity 1,
+ * rain_per_iter 0, mode HMG_VP_FUSED. */
+void hmg_vpipes_defaults(hmg_vpipes_params *p);
+/* Waits for all work queued on the context stream. */
+int hmg_sync(hmg_ctx *ctx);
+/* The HIP stream (hipStream_t) of the context, for callers that enqueue
+ * their own work around the *_dev entry points. */
+void *hmg_stream(hmg_ctx *ctx);
+
+/* -------------------------------------------------------------- noise -- */
+
+/* hmap::gpu::noise_fbm (HighMap/src/primitives/coherent_noise/
+ * primitives_gpu.cpp:543-591; defaults coherent_noise.hpp:821-834:
+ * octaves 8, weight 0.7, persistence 0.5, lacunarity 2, bbox {0,1,0,1},
+ * period {0,0}).  noise_type follows hmap::NoiseType; only PERLIN (1) is
+ * implemented, other types return HMG_ERR_UNSUPPORTED.  ctrl, noise_x,
+ * noise_y may be NULL; bbox and period may be NULL for the defaults. */
+int hmg_noise_fbm(hmg_ctx     *ctx,
+                  float       *out_host,
+                  int          nx,
+                  int          ny,
+                  int          noise_type,
+                  float        kx,
+                  float        ky,
+                  uint32_t     seed,
+                  int          octaves,
+                  float        weight,
+                  float        persistence,
+                  float        lacunarity,
+                  const float *ctrl_host,
+                  const float *noise_x_host,
+                  const float *noise_y_host,
+                  const float *bbox4,
+                  const int   *period2);
+/* Same on device buffers, asynchronous on the context stream. */
+int hmg_noise_fbm_dev(hmg_ctx     *ctx,
+                      float       *out_dev,
+                      int          nx,
+                      int          ny,
+                      int          noise_type,
+                      float        kx,
+                      float        ky,
+                      uint32_t     seed,
+                      int          octaves,
+                      float        weight,
+                      float        persistence,
+                      float        lacunarity,
+                      const float *ctrl_dev,
+                      const float *noise_x_dev,
+                      const float *noise_y_dev,
+                      const float *bbox4,
+                      const int   *period2);
+
+/* hmap::remap(array, vmin, vmax) (HighMap/src/range/range.cpp:342-361) on a
+ * device buffer; min and max are reduced on the device. */
+int hmg_remap_dev(hmg_ctx *ctx, float *a_dev, size_t n, float vmin, float vmax);
+
+/* ------------------------------------------------------------ thermal -- */
+
+/* hmap::gpu::thermal (HighMap/src/erosion/thermal_gpu.cpp:17-96; defaults
+ * HighMap/include/highmap/erosion.hpp:1784-1801, iterations 10).
+ * talus_host == NULL means the scalar overload (talus_scalar everywhere).
+ * bedrock_host selects the thermal_with_bedrock kernel
+ * (thermal.cl:60-100).  deposition_host, if not NULL, receives
+ * max(z - z_before, 0).  mask_host, if not NULL, blends the result as
+ * apply_with_mask does (HighMap/include/highmap/operator.hpp:405-418).
+ * Semantics: Jacobi steps (each step reads step t, writes t+1), then
+ * extrapolate_borders (HighMap/src/boundary/boundary.cpp:19-41).
+ * nx and ny must be >= 3. */
+int hmg_thermal(hmg_ctx     *ctx,
+                float       *z_host,
+                int          nx,
+                int          ny,
+                const float *talus_host,
+                float        talus_scalar,
+                int          iterations,
+                const float *bedrock_host,
+                float       *deposition_host,
+                const float *mask_host);
+/* Device form: `iterations` Jacobi steps then extrapolate_borders, in place
+ * on z_dev, asynchronous on the context stream.  talus_dev / bedrock_dev may
+ * be NULL. */
+int hmg_thermal_dev(hmg_ctx     *ctx,
+                    float       *z_dev,
+                    int          nx,
+                    int          ny,
+                    const float *talus_dev,
+                    float        talus_scalar,
+                    int          iterations,
+                    const float *bedrock_dev);
+
+/* ------------------------------------------------------------- vpipes -- */
+
+/* hmap::gpu::hydraulic_vpipes (HighMap/src/erosion/
+ * hydraulic_vpipes_gpu.cpp:15-178).  z is updated in place; rain_host may be
+ * NULL (uniform 1); d/s/u/v outputs may be NULL.  p == NULL means the
+ * reference defaults. */
+int hmg_vpipes(hmg_ctx                 *ctx,
+               float                   *z_host,
+               int                      nx,
+               int                      ny,
+               const hmg_vpipes_params *p,
+               const float             *rain_host,
+               float                   *d_out_host,
+               float                   *s_out_host,
+               float                   *u_out_host,
+               float                   *v_out_host);
+
+/* ------------------------------------------------------------ session -- */
+
+/* A session keeps one erosion state (z, water, sediment, fluxes) resident
+ * in device memory between calls, so that hydraulic iterations and thermal
+ * steps can be interleaved on one continuous state without host transfers.
+ * The water, sediment and flux planes are stored as `storage`; all
+ * arithmetic is FP32; the water-volume reduction is exact fixed point. */
+int hmg_session_create(hmg_ctx                 *ctx,
+                       const float             *z_host,
+                       int                      nx,
+                       int                      ny,
+                       const hmg_vpipes_params *p,
+                       const float             *rain_host,
+                       int                      storage,
+                       hmg_session            **out);
+/* Same, with z and rain already on the device (copied into the session). */
+int hmg_session_create_dev(hmg_ctx                 *ctx,
+                           const float             *z_dev,
+                           int                      nx,
+                           int                      ny,
+                           const hmg_vpipes_params *p,
+                           const float             *rain_dev,
+                           int                      storage,
+                           hmg_session            **out);
+/* Queues `iterations` hydraulic iterations (asynchronous). */
+int hmg_session_vpipes(hmg_session *s, int iterations);
+/* Queues one thermal() call on the session's z: `iterations` Jacobi steps
+ * with uniform talus, then extrapolate_borders (asynchronous). */
+int hmg_session_thermal(hmg_session *s, float talus, int iterations);
+/* Waits, then copies the state out, in the form hydraulic_vpipes returns it
+ * (d after evaporation and top-up, s after transport, u and v of the last
+ * iteration).  Any pointer may be NULL. */
+int hmg_session_export(hmg_session *s,
+                       float       *z_host,
+                       float       *d_host,
+                       float       *s_host,
+                       float       *u_host,
+                       float       *v_host);
+/* Waits, then returns sum(z), sum(s), sum(d) in FP64 (mass check). */
+int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d);
+/* Device pointer of the session's current z plane (FP32, nx*ny). */
+float *hmg_session_z_dev(hmg_session *s);
+int    hmg_session_sync(hmg_session *s);
+int    hmg_session_destroy(hmg_session *s);
+/* Device bytes a session of this shape and storage holds. */
+size_t hmg_session_bytes(int nx, int ny, int storage);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,461 @@
+/*
+ * hm_oracle.c -- CPU oracle for the HighMap grid-erosion hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see hm_oracle.h).  Built with
+ * -ffp-contract=off and without -ffast-math so every float operation below
+ * is one IEEE-754 operation, rounded to nearest even.
+ *
+ * Noise, hashing and the project-written sin/cos live in this file; thermal
+ * and vpipes are written once in hm_oracle_erosion.inc and instantiated for
+ * float (the parity contract) and double (the error oracle).
+ */
+#include "hm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Integer seed path.                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Thomas Wang's 32-bit integer hash, as used by the reference
+ * (HighMap/src/gpu_opencl/kernels/_common_rand.cl:66-74).
+ * Parity pinned: known-answer values cross-checked against an independent
+ * numpy uint32 restatement (tests/test_oracle_noise.py). */
+uint32_t orc_wang_hash(uint32_t seed)
+{
+  seed = (seed ^ 61u) ^ (seed >> 16);
+  seed *= 9u;
+  seed = seed ^ (seed >> 4);
+  seed *= 0x27d4eb2du;
+  seed = seed ^ (seed >> 15);
+  return seed;
+}
+
+/* Marsaglia xorshift32 with shifts (13, 17, 5)
+ * (HighMap/src/gpu_opencl/kernels/_common_rand.cl:76-82).
+ * Parity pinned: full period 2^32-1 checked by brute force, plus numpy. */
+uint32_t orc_xorshift32(uint32_t x)
+{
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  return x;
+}
+
+/* fseed = rand(wang_hash(seed)): one xorshift step, then u32 * 2^-32 in float
+ * (HighMap/src/gpu_opencl/kernels/_common_rand.cl:84-88, noise.cl:483-484). */
+float orc_seed_to_fseed(uint32_t seed)
+{
+  uint32_t st = orc_xorshift32(orc_wang_hash(seed));
+  return (float)st * (1.f / 4294967296.f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Project-written sin / cos.                                                */
+/*                                                                           */
+/* Method: Cody-Waite argument reduction by pi/2 with a three-part constant  */
+/* (each part the float nearest the remaining tail of pi/2), every step an   */
+/* explicit fused multiply-add, then the Taylor polynomials of sin (to r^9)  */
+/* and cos (to r^10) on |r| <= pi/4, evaluated by Horner with fmaf.  The    */
+/* reference uses the OpenCL built-in sin/cos under -cl-fast-relaxed-math,   */
+/* whose results are vendor-defined; a shared, fully specified routine is    */
+/* what lets the GPU noise match this restatement bit for bit.               */
+/* Parity pinned: error against numpy float64 sin/cos over |x| <= 2^18       */
+/* (tests/test_oracle_noise.py).                                             */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_TWO_OVER_PI 0x1.45f306p-1f
+#define ORC_PIO2_HI 0x1.921fb6p+0f
+#define ORC_PIO2_MID -0x1.777a5cp-25f
+#define ORC_PIO2_LO -0x1.ee59dap-50f
+
+static float orc_reduce(float x, int *quadrant)
+{
+  float k = rintf(x * ORC_TWO_OVER_PI);
+  float r = fmaf(-k, ORC_PIO2_HI, x);
+  r = fmaf(-k, ORC_PIO2_MID, r);
+  r = fmaf(-k, ORC_PIO2_LO, r);
+  *quadrant = ((int)k) & 3;
+  return r;
+}
+
+static float orc_sin_poly(float r)
+{
+  float r2 = r * r;
+  float p = fmaf(r2, 1.f / 362880.f, -1.f / 5040.f);
+  p = fmaf(r2, p, 1.f / 120.f);
+  p = fmaf(r2, p, -1.f / 6.f);
+  return fmaf(r * r2, p, r);
+}
+
+static float orc_cos_poly(float r)
+{
+  float r2 = r * r;
+  float p = fmaf(r2, -1.f / 3628800.f, 1.f / 40320.f);
+  p = fmaf(r2, p, -1.f / 720.f);
+  p = fmaf(r2, p, 1.f / 24.f);
+  p = fmaf(r2, p, -0.5f);
+  return fmaf(r2, p, 1.f);
+}
+
+float orc_sinf(float x)
+{
+  int   q;
+  float r = orc_reduce(x, &q);
+  switch (q)
+  {
+  case 0: return orc_sin_poly(r);
+  case 1: return orc_cos_poly(r);
+  case 2: return -orc_sin_poly(r);
+  default: return -orc_cos_poly(r);
+  }
+}
+
+float orc_cosf(float x)
+{
+  int   q;
+  float r = orc_reduce(x, &q);
+  switch (q)
+  {
+  case 0: return orc_cos_poly(r);
+  case 1: return -orc_sin_poly(r);
+  case 2: return -orc_cos_poly(r);
+  default: return orc_sin_poly(r);
+  }
+}
+
+/* OpenCL fract(): min(x - floor(x), 0x1.fffffep-1f). */
+float orc_fractf(float x)
+{
+  return fminf(x - floorf(x), 0x1.fffffep-1f);
+}
+
+/* hash12f (HighMap/src/gpu_opencl/kernels/_common_rand.cl:12-17):
+ * fract(sin(dot(p, (127.1, 311.7)) + fseed) * 43758.5453123).  The dot
+ * product is two rounded products and one rounded sum (no contraction). */
+float orc_hash12f(float px, float py, float fseed)
+{
+  float dp = px * 127.1f + py * 311.7f;
+  return orc_fractf(orc_sinf(dp + fseed) * 43758.5453123f);
+}
+
+/* wrap_lattice (noise.cl:12-17). */
+static float orc_wrap(float c, int period)
+{
+  if (period > 0) c = c - (float)period * floorf(c / (float)period);
+  return c;
+}
+
+/* scale_period (noise.cl:21-26). */
+static int orc_scale_period(int period, float lacunarity)
+{
+  if (period > 0) period = (int)((float)period * lacunarity + 0.5f);
+  return period;
+}
+
+/* grad22f (_common_rand.cl:53-57) at a (wrapped) lattice corner, then the
+ * dot product with the offset (dx, dy). */
+static float orc_corner(float cx, float cy, float fseed, float dx, float dy)
+{
+  float angle = 6.2831853f * orc_hash12f(cx, cy, fseed);
+  float gx = orc_cosf(angle);
+  float gy = orc_sinf(angle);
+  return gx * dx + gy * dy;
+}
+
+static float orc_fade(float x)
+{
+  /* smoothstep5 (_common_math.cl:126-129): x*x*x*(x*(6x-15)+10) */
+  return x * x * x * (x * (6.f * x - 15.f) + 10.f);
+}
+
+static float orc_lerp(float a, float b, float t)
+{
+  /* lerp (_common_math.cl:47-50) */
+  return (1.f - t) * a + t * b;
+}
+
+/* base_perlin (noise.cl:28-63).
+ * Parity pinned: zero at lattice points, bound 1.42857*sqrt(2)/2,
+ * continuity across cell edges, tiling with a period, numpy restatement. */
+float orc_base_perlin(float px, float py, float fseed, int per_x, int per_y)
+{
+  float ix = floorf(px), iy = floorf(py);
+  float fx = orc_fractf(px), fy = orc_fractf(py);
+
+  float x0 = orc_wrap(ix, per_x), x1 = orc_wrap(ix + 1.f, per_x);
+  float y0 = orc_wrap(iy, per_y), y1 = orc_wrap(iy + 1.f, per_y);
+
+  float n00 = orc_corner(x0, y0, fseed, fx, fy);
+  float n10 = orc_corner(x1, y0, fseed, fx - 1.f, fy);
+  float n01 = orc_corner(x0, y1, fseed, fx, fy - 1.f);
+  float n11 = orc_corner(x1, y1, fseed, fx - 1.f, fy - 1.f);
+
+  float ux = orc_fade(fx), uy = orc_fade(fy);
+  float nx0 = orc_lerp(n00, n10, ux);
+  float nx1 = orc_lerp(n01, n11, ux);
+  return 1.42857f * orc_lerp(nx0, nx1, uy);
+}
+
+/* base_perlin_fbm (noise.cl:194-216).
+ * Parity pinned: octaves=1 gives 0.6*P(pos); weight=0 gives amplitudes
+ * 0.6*pers^k (closed forms in tests/test_oracle_noise.py). */
+static float orc_perlin_fbm(float px,
+                            float py,
+                            int   octaves,
+                            float weight,
+                            float persistence,
+                            float lacunarity,
+                            float fseed,
+                            int   per_x,
+                            int   per_y)
+{
+  float n = 0.f, nf = 1.f, na = 0.6f;
+  for (int k = 0; k < octaves; k++)
+  {
+    float v = orc_base_perlin(px * nf, py * nf, fseed, per_x, per_y);
+    n += v * na;
+    na *= (1.f - weight) + weight * fminf(v + 1.f, 2.f) * 0.5f;
+    na *= persistence;
+    nf *= lacunarity;
+    per_x = orc_scale_period(per_x, lacunarity);
+    per_y = orc_scale_period(per_y, lacunarity);
+  }
+  return n;
+}
+
+/* noise_fbm kernel, PERLIN branch (noise.cl:456-504) with g_to_xy
+ * (_common_index.cl:8-24); host defaults from
+ * HighMap/include/highmap/primitives/coherent_noise.hpp:821-834. */
+int orc_noise_fbm_f32(float       *out,
+                      int          nx,
+                      int          ny,
+                      int          noise_type,
+                      float        kx,
+                      float        ky,
+                      uint32_t     seed,
+                      int          octaves,
+                      float        weight,
+                      float        persistence,
+                      float        lacunarity,
+                      const float *ctrl,
+                      const float *noise_x,
+                      const float *noise_y,
+                      const float *bbox4,
+                      const int   *period2)
+{
+  if (noise_type != 1) return -1;
+  const float bbox_def[4] = {0.f, 1.f, 0.f, 1.f};
+  const int   per_def[2] = {0, 0};
+  if (!bbox4) bbox4 = bbox_def;
+  if (!period2) period2 = per_def;
+
+  const float fseed = orc_seed_to_fseed(seed);
+
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++)
+    {
+      size_t idx = (size_t)j * nx + i;
+      float  ct = ctrl ? ctrl[idx] : 1.f;
+      float  dx = noise_x ? noise_x[idx] : 0.f;
+      float  dy = noise_y ? noise_y[idx] : 0.f;
+      float  w = (1.f - ct) + ct * weight;
+
+      float x = (float)i / (float)nx;
+      float y = (float)j / (float)ny;
+      x = kx * (x * (bbox4[1] - bbox4[0]) + bbox4[0]) + kx * dx;
+      y = ky * (y * (bbox4[3] - bbox4[2]) + bbox4[2]) + ky * dy;
+
+      out[idx] = orc_perlin_fbm(x,
+                                y,
+                                octaves,
+                                w,
+                                persistence,
+                                lacunarity,
+                                fseed,
+                                period2[0],
+                                period2[1]);
+    }
+  return 0;
+}
+
+/* remap to [vmin, vmax] (HighMap/src/range/range.cpp:342-361). */
+void orc_remap_f32(float *a, size_t n, float vmin, float vmax)
+{
+  if (n == 0) return;
+  float mn = a[0], mx = a[0];
+  for (size_t k = 1; k < n; k++)
+  {
+    if (a[k] < mn) mn = a[k];
+    if (a[k] > mx) mx = a[k];
+  }
+  if (mn == mx)
+  {
+    for (size_t k = 0; k < n; k++) a[k] = vmin;
+    return;
+  }
+  const float scale = (vmax - vmin) / (mx - mn);
+  const float offset = vmin - mn * scale;
+  for (size_t k = 0; k < n; k++) a[k] = a[k] * scale + offset;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Order-independent water-volume sum.                                       */
+/*                                                                           */
+/* Each float term x is mapped to the integer round(x * 2^64) (exact for    */
+/* |x| >= 2^-41, round-to-nearest-even below) and the terms are summed as    */
+/* 128-bit two's-complement integers, so the result does not depend on the   */
+/* summation order, the tile shape or the thread count.  The 128-bit sum is  */
+/* turned into a double by explicit 32-bit pieces so that the device code    */
+/* can repeat the same roundings.  The reference sums in FP32 with           */
+/* std::accumulate (HighMap/src/array/methods.cpp:428-431).                  */
+/* Parity pinned: exact against Python integer arithmetic on random and      */
+/* edge-case inputs (tests/test_oracle_vpipes.py).                           */
+/* ------------------------------------------------------------------------ */
+
+typedef __int128          i128;
+typedef unsigned __int128 u128;
+
+static i128 orc_fix_term(float x)
+{
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  uint32_t e = (b >> 23) & 0xffu;
+  uint32_t m = b & 0x7fffffu;
+  if (e == 0) return 0; /* zero or subnormal: |x| * 2^64 < 2^-62 */
+  m |= 0x800000u;
+  int  sh = (int)e - 86; /* x = m * 2^(e-150); x * 2^64 = m * 2^(e-86) */
+  u128 mag;
+  if (sh >= 0)
+    mag = (u128)m << (sh > 103 ? 103 : sh);
+  else if (sh > -25)
+  {
+    uint32_t r = (uint32_t)(-sh);
+    uint32_t q = m >> r;
+    uint32_t rem = m & ((1u << r) - 1u);
+    uint32_t half = 1u << (r - 1);
+    if (rem > half || (rem == half && (q & 1u))) q++;
+    mag = q;
+  }
+  else
+    mag = 0;
+  return (b >> 31) ? -(i128)mag : (i128)mag;
+}
+
+static double orc_u32d(uint32_t x)
+{
+  return (double)x;
+}
+
+static double orc_fix_to_double(i128 s)
+{
+  u128     us = (u128)s;
+  uint64_t lo = (uint64_t)us;
+  int64_t  hi = (int64_t)(uint64_t)(us >> 64);
+  /* hi as double: (hi >> 32) * 2^32 + (hi & 0xffffffff), one rounding */
+  double hid = (double)(int32_t)(hi >> 32) * 4294967296.0 +
+               orc_u32d((uint32_t)(uint64_t)hi);
+  double lod = orc_u32d((uint32_t)(lo >> 32)) * 4294967296.0 +
+               orc_u32d((uint32_t)lo);
+  return hid + lod * 0x1p-64;
+}
+
+double orc_fixsum_f32(const float *a, size_t n)
+{
+  i128 s = 0;
+  for (size_t k = 0; k < n; k++) s += orc_fix_term(a[k]);
+  return orc_fix_to_double(s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Thermal and vpipes, instantiated for float and double.                    */
+/* ------------------------------------------------------------------------ */
+
+static float orc_hypotf(float x, float y)
+{
+  /* hypot convention: sqrt(x*x + y*y), three IEEE operations. */
+  return sqrtf(x * x + y * y);
+}
+
+static double orc_hypotd(double x, double y)
+{
+  return sqrt(x * x + y * y);
+}
+
+static float orc_powf(float x, float e)
+{
+  return e == 1.f ? x : powf(x, e);
+}
+
+static double orc_powd(double x, double e)
+{
+  return e == 1.0 ? x : pow(x, e);
+}
+
+static double orc_sum_f64(const double *a, int nx, int ny)
+{
+  /* per-row partial sums, then rows in order: independent of thread count */
+  double *rows = (double *)malloc(sizeof(double) * (size_t)ny);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++)
+  {
+    double s = 0.0;
+    for (int i = 0; i < nx; i++) s += a[(size_t)j * nx + i];
+    rows[j] = s;
+  }
+  double s = 0.0;
+  for (int j = 0; j < ny; j++) s += rows[j];
+  free(rows);
+  return s;
+}
+
+static double orc_sum_f32fix(const float *a, int nx, int ny)
+{
+  return orc_fixsum_f32(a, (size_t)nx * (size_t)ny);
+}
+
+#define REAL float
+#define SFX(name) name##_f32
+#define RC(x) (x##f)
+#define R_SQRT sqrtf
+#define R_FLOOR floorf
+#define R_RINT rintf
+#define R_FABS fabsf
+#define R_MAX fmaxf
+#define R_MIN fminf
+#define R_HYPOT orc_hypotf
+#define R_POW orc_powf
+#define R_VOLUME orc_sum_f32fix
+#define R_IS_F32 1
+#include "hm_oracle_erosion.inc"
+#undef REAL
+#undef SFX
+#undef RC
+#undef R_SQRT
+#undef R_FLOOR
+#undef R_RINT
+#undef R_FABS
+#undef R_MAX
+#undef R_MIN
+#undef R_HYPOT
+#undef R_POW
+#undef R_VOLUME
+#undef R_IS_F32
+
+#define REAL double
+#define SFX(name) name##_f64
+#define RC(x) ((double)(x##f))
+#define R_SQRT sqrt
+#define R_FLOOR floor
+#define R_RINT rint
+#define R_FABS fabs
+#define R_MAX fmax
+#define R_MIN fmin
+#define R_HYPOT orc_hypotd
+#define R_POW orc_powd
+#define R_VOLUME orc_sum_f64
+#define R_IS_F32 0
+#include "hm_oracle_erosion.inc"

@@ -1,0 +1,195 @@
+/*
+ * hm_oracle.h -- CPU oracle for the HighMap grid-erosion hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (highmap_amd/, include/)
+ * links, loads or calls this code.  It is used by tests/, by the smoke check
+ * and by the bench's cpu_baseline leg, always as the checker, never as the
+ * thing measured or shipped.
+ *
+ * Two restatements of the reference algorithm live here:
+ *   *_f32 : the FP32 parity contract.  Same operation order as the HIP
+ *           kernels, no floating-point contraction (built with
+ *           -ffp-contract=off), correctly rounded division and sqrt,
+ *           project-written sin/cos.  The GPU must match it bit for bit.
+ *   *_f64 : the FP64 error oracle.  The same algorithm with every float
+ *           constant of the reference promoted to double.  Used to state the
+ *           error of every FP32/BF16/FP16 variant.
+ *
+ * Reference files followed (read as text, HighMap source tree):
+ *   HighMap/src/gpu_opencl/kernels/noise.cl:12-63,194-216,456-504
+ *   HighMap/src/gpu_opencl/kernels/_common_rand.cl:12-17,53-57,66-88
+ *   HighMap/src/gpu_opencl/kernels/_common_index.cl:8-24,88-132
+ *   HighMap/src/gpu_opencl/kernels/_common_math.cl:47-50,126-129
+ *   HighMap/src/gpu_opencl/kernels/thermal.cl:5-58
+ *   HighMap/src/gpu_opencl/kernels/hydraulic_vpipes.cl:5-211
+ *   HighMap/src/erosion/hydraulic_vpipes_gpu.cpp:34-177
+ *   HighMap/src/erosion/thermal_gpu.cpp:17-96
+ *   HighMap/src/boundary/boundary.cpp:19-41
+ *   HighMap/src/range/range.cpp:342-361
+ *
+ * Parity status of each function is stated in its comment in hm_oracle.c.
+ */
+#ifndef HM_ORACLE_H
+#define HM_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parameters of one vpipes run.  Field meaning follows
+ * hmap::gpu::hydraulic_vpipes (HighMap/include/highmap/erosion.hpp:1206-1223);
+ * gravity and rain_per_iter are extensions (reference values: 1 and 0). */
+typedef struct orc_vp_params
+{
+  float water_height;
+  float evap_rate;
+  float dt;
+  float k_capacity;
+  float k_erode;
+  float k_depose;
+  float k_discharge_exp;
+  float downcutting_max_depth_ratio;
+  float flux_diffusion_strength;
+  float gravity;
+  float rain_per_iter;
+  int   maintain_water_volume;
+  int   flux_diffusion;
+} orc_vp_params;
+
+/* --- integer seed path and project-written transcendental functions --- */
+uint32_t orc_wang_hash(uint32_t seed);
+uint32_t orc_xorshift32(uint32_t state);
+float    orc_seed_to_fseed(uint32_t seed);
+float    orc_sinf(float x);
+float    orc_cosf(float x);
+float    orc_fractf(float x);
+float    orc_hash12f(float px, float py, float fseed);
+float    orc_base_perlin(float px, float py, float fseed, int per_x, int per_y);
+
+/* --- noise_fbm (PERLIN branch only); returns 0, or -1 for other types --- */
+int orc_noise_fbm_f32(float       *out,
+                      int          nx,
+                      int          ny,
+                      int          noise_type,
+                      float        kx,
+                      float        ky,
+                      uint32_t     seed,
+                      int          octaves,
+                      float        weight,
+                      float        persistence,
+                      float        lacunarity,
+                      const float *ctrl,
+                      const float *noise_x,
+                      const float *noise_y,
+                      const float *bbox4,
+                      const int   *period2);
+
+void orc_remap_f32(float *a, size_t n, float vmin, float vmax);
+
+/* --- thermal --- */
+void orc_thermal_steps_f32(float       *z,
+                           int          nx,
+                           int          ny,
+                           const float *talus,
+                           float        talus_scalar,
+                           int          iterations);
+void orc_thermal_steps_f64(double       *z,
+                           int           nx,
+                           int           ny,
+                           const double *talus,
+                           double        talus_scalar,
+                           int           iterations);
+void orc_extrapolate_borders_f32(float *z, int nx, int ny);
+void orc_extrapolate_borders_f64(double *z, int nx, int ny);
+
+/* --- fixed-point water volume (order-independent, exact per term) --- */
+double orc_fixsum_f32(const float *a, size_t n);
+
+/* --- vpipes --- */
+/* Initial state: d = water_height * rain, s = fl = fr = ft = fb = u = v = 0.
+ * Writes W0 = sum(d) and R = sum(rain) (rain==NULL means all ones). */
+void orc_vpipes_init_f32(float         *d,
+                         float         *s,
+                         float         *fl,
+                         float         *fr,
+                         float         *ft,
+                         float         *fb,
+                         float         *u,
+                         float         *v,
+                         int            nx,
+                         int            ny,
+                         const orc_vp_params *p,
+                         const float   *rain,
+                         double        *W0,
+                         double        *R);
+void orc_vpipes_init_f64(double        *d,
+                         double        *s,
+                         double        *fl,
+                         double        *fr,
+                         double        *ft,
+                         double        *fb,
+                         double        *u,
+                         double        *v,
+                         int            nx,
+                         int            ny,
+                         const orc_vp_params *p,
+                         const double  *rain,
+                         double        *W0,
+                         double        *R);
+/* Runs `iterations` full reference iterations (flow, water, erosion,
+ * transport, evaporation, volume top-up) on the caller-owned state. */
+void orc_vpipes_run_f32(float         *z,
+                        float         *d,
+                        float         *s,
+                        float         *fl,
+                        float         *fr,
+                        float         *ft,
+                        float         *fb,
+                        float         *u,
+                        float         *v,
+                        int            nx,
+                        int            ny,
+                        const orc_vp_params *p,
+                        const float   *rain,
+                        double         W0,
+                        double         R,
+                        int            iterations);
+void orc_vpipes_run_f64(double        *z,
+                        double        *d,
+                        double        *s,
+                        double        *fl,
+                        double        *fr,
+                        double        *ft,
+                        double        *fb,
+                        double        *u,
+                        double        *v,
+                        int            nx,
+                        int            ny,
+                        const orc_vp_params *p,
+                        const double  *rain,
+                        double         W0,
+                        double         R,
+                        int            iterations);
+
+/* Single-pass entry points, for pass-level tests. */
+void orc_vp_flow_f32(const float *z, const float *d1, const float *fl,
+                     const float *fr, const float *ft, const float *fb,
+                     float *flo, float *fro, float *fto, float *fbo, int nx,
+                     int ny, const orc_vp_params *p);
+void orc_vp_water_f32(const float *fl, const float *fr, const float *ft,
+                      const float *fb, const float *d1, float *d2, float *u,
+                      float *v, int nx, int ny, const orc_vp_params *p);
+void orc_vp_erosion_f32(const float *z, const float *s, const float *d2,
+                        const float *u, const float *v, float *zo, float *so,
+                        int nx, int ny, const orc_vp_params *p);
+void orc_vp_transport_f32(const float *u, const float *v, const float *s,
+                          float *so, int nx, int ny, float dt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

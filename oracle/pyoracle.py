@@ -1,0 +1,240 @@
+"""ctypes binding of liboracle.so (hm_oracle.c).  TEST INFRASTRUCTURE ONLY.
+
+Arrays follow hmap::Array: shape (ny, nx) in numpy terms, C order, so the
+linear index is j*nx + i with i the fastest axis
+(HighMap/include/highmap/array.hpp:236-239).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class VpParams(C.Structure):
+    """Mirror of orc_vp_params (and of hmg_vpipes_params' leading fields)."""
+
+    _fields_ = [
+        ("water_height", C.c_float),
+        ("evap_rate", C.c_float),
+        ("dt", C.c_float),
+        ("k_capacity", C.c_float),
+        ("k_erode", C.c_float),
+        ("k_depose", C.c_float),
+        ("k_discharge_exp", C.c_float),
+        ("downcutting_max_depth_ratio", C.c_float),
+        ("flux_diffusion_strength", C.c_float),
+        ("gravity", C.c_float),
+        ("rain_per_iter", C.c_float),
+        ("maintain_water_volume", C.c_int),
+        ("flux_diffusion", C.c_int),
+    ]
+
+
+# Defaults of hmap::gpu::hydraulic_vpipes (erosion.hpp:1206-1223).
+VP_DEFAULTS = dict(
+    water_height=1e-2,
+    evap_rate=0.1,
+    dt=0.5,
+    k_capacity=0.5,
+    k_erode=1e-3,
+    k_depose=1e-2,
+    k_discharge_exp=1.0,
+    downcutting_max_depth_ratio=10.0,
+    flux_diffusion_strength=0.01,
+    gravity=1.0,
+    rain_per_iter=0.0,
+    maintain_water_volume=1,
+    flux_diffusion=1,
+)
+
+
+def vp_params(**kw):
+    d = dict(VP_DEFAULTS)
+    for k, v in kw.items():
+        if k not in d:
+            raise KeyError(k)
+        d[k] = v
+    return VpParams(**{k: (int(v) if k in ("maintain_water_volume", "flux_diffusion") else float(v))
+                       for k, v in d.items()})
+
+
+def build():
+    """Compile liboracle.so and cpu_bench with oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    u32 = C.c_uint32
+    L.orc_wang_hash.argtypes = [u32]
+    L.orc_wang_hash.restype = u32
+    L.orc_xorshift32.argtypes = [u32]
+    L.orc_xorshift32.restype = u32
+    L.orc_seed_to_fseed.argtypes = [u32]
+    L.orc_seed_to_fseed.restype = C.c_float
+    for name in ("orc_sinf", "orc_cosf", "orc_fractf"):
+        getattr(L, name).argtypes = [C.c_float]
+        getattr(L, name).restype = C.c_float
+    L.orc_hash12f.argtypes = [C.c_float, C.c_float, C.c_float]
+    L.orc_hash12f.restype = C.c_float
+    L.orc_base_perlin.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
+    L.orc_base_perlin.restype = C.c_float
+    L.orc_noise_fbm_f32.argtypes = [F32P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                    u32, C.c_int, C.c_float, C.c_float, C.c_float,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_noise_fbm_f32.restype = C.c_int
+    L.orc_remap_f32.argtypes = [F32P, C.c_size_t, C.c_float, C.c_float]
+    L.orc_thermal_steps_f32.argtypes = [F32P, C.c_int, C.c_int, C.c_void_p, C.c_float, C.c_int]
+    L.orc_thermal_steps_f64.argtypes = [F64P, C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int]
+    L.orc_extrapolate_borders_f32.argtypes = [F32P, C.c_int, C.c_int]
+    L.orc_extrapolate_borders_f64.argtypes = [F64P, C.c_int, C.c_int]
+    L.orc_fixsum_f32.argtypes = [F32P, C.c_size_t]
+    L.orc_fixsum_f32.restype = C.c_double
+    pp = C.POINTER(VpParams)
+    dp = C.POINTER(C.c_double)
+    L.orc_vpipes_init_f32.argtypes = [F32P] * 8 + [C.c_int, C.c_int, pp, C.c_void_p, dp, dp]
+    L.orc_vpipes_init_f64.argtypes = [F64P] * 8 + [C.c_int, C.c_int, pp, C.c_void_p, dp, dp]
+    L.orc_vpipes_run_f32.argtypes = [F32P] * 9 + [C.c_int, C.c_int, pp, C.c_void_p,
+                                                  C.c_double, C.c_double, C.c_int]
+    L.orc_vpipes_run_f64.argtypes = [F64P] * 9 + [C.c_int, C.c_int, pp, C.c_void_p,
+                                                  C.c_double, C.c_double, C.c_int]
+    L.orc_vp_flow_f32.argtypes = [F32P] * 10 + [C.c_int, C.c_int, pp]
+    L.orc_vp_water_f32.argtypes = [F32P] * 8 + [C.c_int, C.c_int, pp]
+    L.orc_vp_erosion_f32.argtypes = [F32P] * 7 + [C.c_int, C.c_int, pp]
+    L.orc_vp_transport_f32.argtypes = [F32P] * 4 + [C.c_int, C.c_int, C.c_float]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ----------------------------------------------------------------- noise
+
+
+def noise_fbm(nx, ny, kw=(4.0, 4.0), seed=0, octaves=8, weight=0.7, persistence=0.5,
+              lacunarity=2.0, ctrl=None, noise_x=None, noise_y=None,
+              bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0), noise_type=1):
+    """FP32 restatement of hmap::gpu::noise_fbm (PERLIN)."""
+    out = np.zeros((ny, nx), np.float32)
+    keep = []
+
+    def opt(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.shape == (ny, nx)
+        keep.append(a)
+        return _ptr(a)
+
+    bb = np.asarray(bbox, np.float32)
+    pe = np.asarray(period, np.int32)
+    rc = lib().orc_noise_fbm_f32(out, nx, ny, noise_type, kw[0], kw[1], seed & 0xFFFFFFFF,
+                                 octaves, weight, persistence, lacunarity,
+                                 opt(ctrl), opt(noise_x), opt(noise_y), _ptr(bb), _ptr(pe))
+    if rc != 0:
+        raise ValueError("unsupported noise type %d" % noise_type)
+    return out
+
+
+def remap(a, vmin=0.0, vmax=1.0):
+    a = np.ascontiguousarray(a, np.float32).copy()
+    lib().orc_remap_f32(a.reshape(-1), a.size, vmin, vmax)
+    return a
+
+
+def fbm_terrain(n_x, n_y, seed):
+    """remap(noise_fbm(PERLIN, kw 4, seed, 8 octaves, w 0.7, pers 0.5, lac 2)) in [0,1]."""
+    return remap(noise_fbm(n_x, n_y, (4.0, 4.0), seed))
+
+
+# --------------------------------------------------------------- thermal
+
+
+def thermal_steps(z, talus, iterations, dtype=np.float32):
+    z = np.ascontiguousarray(z, dtype).copy()
+    ny, nx = z.shape
+    L = lib()
+    fn = L.orc_thermal_steps_f32 if dtype == np.float32 else L.orc_thermal_steps_f64
+    if np.isscalar(talus):
+        fn(z, nx, ny, None, float(talus), iterations)
+    else:
+        t = np.ascontiguousarray(talus, dtype)
+        fn(z, nx, ny, _ptr(t), 0.0, iterations)
+    return z
+
+
+def extrapolate_borders(z, dtype=np.float32):
+    z = np.ascontiguousarray(z, dtype).copy()
+    ny, nx = z.shape
+    L = lib()
+    (L.orc_extrapolate_borders_f32 if dtype == np.float32 else L.orc_extrapolate_borders_f64)(z, nx, ny)
+    return z
+
+
+def thermal(z, talus, iterations=10, dtype=np.float32):
+    """hmap::gpu::thermal restated: Jacobi steps then extrapolate_borders."""
+    return extrapolate_borders(thermal_steps(z, talus, iterations, dtype), dtype)
+
+
+def fixsum(a):
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    return lib().orc_fixsum_f32(a, a.size)
+
+
+# ---------------------------------------------------------------- vpipes
+
+PLANES = ("z", "d", "s", "fl", "fr", "ft", "fb", "u", "v")
+
+
+class VpState:
+    """Caller-owned vpipes state planes for the oracle."""
+
+    def __init__(self, z, params, rain=None, dtype=np.float32):
+        self.dtype = dtype
+        self.z = np.ascontiguousarray(z, dtype).copy()
+        ny, nx = self.z.shape
+        self.nx, self.ny = nx, ny
+        for name in PLANES[1:]:
+            setattr(self, name, np.zeros((ny, nx), dtype))
+        self.p = params
+        self.rain = None if rain is None else np.ascontiguousarray(rain, dtype)
+        W0 = C.c_double()
+        R = C.c_double()
+        L = lib()
+        fn = L.orc_vpipes_init_f32 if dtype == np.float32 else L.orc_vpipes_init_f64
+        fn(self.d, self.s, self.fl, self.fr, self.ft, self.fb, self.u, self.v, nx, ny,
+           C.byref(self.p), _ptr(self.rain), C.byref(W0), C.byref(R))
+        self.W0, self.R = W0.value, R.value
+
+    def run(self, iterations):
+        L = lib()
+        fn = L.orc_vpipes_run_f32 if self.dtype == np.float32 else L.orc_vpipes_run_f64
+        fn(self.z, self.d, self.s, self.fl, self.fr, self.ft, self.fb, self.u, self.v,
+           self.nx, self.ny, C.byref(self.p), _ptr(self.rain), self.W0, self.R, iterations)
+        return self
+
+    def thermal_step(self, talus, iterations=1):
+        """One thermal() call on z of the running state (steps + borders)."""
+        self.z = thermal(self.z, talus, iterations, self.dtype)
+        return self
+
+
+def hydraulic_vpipes(z, params=None, rain=None, iterations=50, dtype=np.float32):
+    """hmap::gpu::hydraulic_vpipes restated; returns (z, d, s, u, v)."""
+    st = VpState(z, params or vp_params(), rain, dtype).run(iterations)
+    return st.z, st.d, st.s, st.u, st.v

@@ -1,0 +1,75 @@
+"""The C ABI library loads and exports every entry point include/*.h declares.
+CPU-only: nothing here launches device work."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(hmg_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_core_entry_points():
+    names = _declared("hmg.h")
+    for n in ("hmg_init", "hmg_noise_fbm", "hmg_thermal", "hmg_vpipes", "hmg_session_create",
+              "hmg_session_vpipes", "hmg_session_thermal", "hmg_session_export",
+              "hmg_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from highmap_amd import LIB_PATH
+
+    assert os.path.exists(LIB_PATH), "libhmg.so not built"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (hmg_[a-z0-9_]+)$", out, flags=re.M))
+    missing = [n for n in _declared("hmg.h") if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_answers_without_gpu():
+    import highmap_amd
+
+    L = highmap_amd.lib()
+    assert L.hmg_abi_version() == 1
+    p = highmap_amd.VpipesParams()
+    L.hmg_vpipes_defaults(ctypes.byref(p))
+    # defaults of hmap::gpu::hydraulic_vpipes (erosion.hpp:1206-1223)
+    f32 = lambda x: float(np.float32(x))
+    assert p.water_height == f32(1e-2) and p.maintain_water_volume == 1
+    assert p.evap_rate == f32(0.1) and p.iterations == 50 and p.dt == 0.5
+    assert p.k_capacity == 0.5 and p.k_erode == f32(1e-3) and p.k_depose == f32(1e-2)
+    assert p.k_discharge_exp == 1.0 and p.downcutting_max_depth_ratio == 10.0
+    assert p.flux_diffusion == 1 and p.flux_diffusion_strength == f32(0.01)
+    assert p.gravity == 1.0 and p.rain_per_iter == 0.0 and p.mode == 0
+
+
+def test_session_bytes_accounting():
+    import highmap_amd
+
+    L = highmap_amd.lib()
+    n = 4096 * 4096
+    assert L.hmg_session_bytes(4096, 4096, 0) >= 2 * n * (4 + 6 * 4)
+    assert L.hmg_session_bytes(4096, 4096, 1) < L.hmg_session_bytes(4096, 4096, 0)
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    """Without a GPU, init reports a device error; nothing computes on the CPU."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import highmap_amd
+
+    with pytest.raises(highmap_amd.HmgError):
+        highmap_amd.Context(0)
