@@ -208,6 +208,10 @@ struct VpConst
   float ce;       // 1 - dt * evap_rate
   float rr;       // rain_per_iter (extension)
   int   maintain; // maintain_water_volume
+  // p / nx in the sampler is a multiplication by 2^-k when nx = 2^k (the
+  // same correctly rounded value); otherwise an IEEE division.
+  float inv_nx, inv_ny;
+  int   pow2x, pow2y;
 };
 
 // hydraulic_vpipes_flow_pass (hydraulic_vpipes.cl:33-68) for one cell.
@@ -346,10 +350,16 @@ __device__ __forceinline__ void vp_erosion(float        zc,
 
 // One axis of the linear mirrored-repeat sampler (OpenCL 1.2 rule for
 // CLK_ADDRESS_MIRRORED_REPEAT | CLK_FILTER_LINEAR applied to p/w).
-__device__ __forceinline__ void mirror_axis(float p, int w, int &i0, int &i1, float &a)
+__device__ __forceinline__ void mirror_axis(float p,
+                                            int   w,
+                                            float inv_w,
+                                            int   pow2,
+                                            int  &i0,
+                                            int  &i1,
+                                            float &a)
 {
   float fw = (float)w;
-  float sg = p / fw;
+  float sg = pow2 ? p * inv_w : p / fw;
   float sp = fabsf(sg - 2.f * __builtin_rintf(0.5f * sg));
   float q = sp * fw - 0.5f;
   float q0 = floorf(q);
@@ -373,8 +383,8 @@ __device__ __forceinline__ float vp_transport(const typename S::T *sp,
   float py = (float)j - k.dt * v;
   int   i0, i1, j0, j1;
   float a, b;
-  mirror_axis(px, k.nx, i0, i1, a);
-  mirror_axis(py, k.ny, j0, j1, b);
+  mirror_axis(px, k.nx, k.inv_nx, k.pow2x, i0, i1, a);
+  mirror_axis(py, k.ny, k.inv_ny, k.pow2y, j0, j1, b);
   float  w00 = (1.f - a) * (1.f - b);
   float  w10 = a * (1.f - b);
   float  w01 = (1.f - a) * b;

@@ -126,6 +126,10 @@ VpConst make_vp_const(const hmg_vpipes_params &p, int nx, int ny)
   k.ce = 1.f - p.dt * p.evap_rate;
   k.rr = p.rain_per_iter;
   k.maintain = p.maintain_water_volume ? 1 : 0;
+  k.pow2x = (nx & (nx - 1)) == 0;
+  k.pow2y = (ny & (ny - 1)) == 0;
+  k.inv_nx = k.pow2x ? 1.f / (float)nx : 0.f;
+  k.inv_ny = k.pow2y ? 1.f / (float)ny : 0.f;
   return k;
 }
 

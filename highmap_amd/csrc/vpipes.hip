@@ -251,171 +251,215 @@ __global__ void __launch_bounds__(256) k_vp_sum(const typename S::T *__restrict_
 }
 
 // ------------------------------------------------------------ fused form
+//
+// One launch per iteration.  Each wavefront owns a strip of 64 columns
+// (60 written, a 2-cell halo on each side) and a chunk of rows, and streams
+// down the rows keeping a rolling window in registers:
+//   A(r)  staged state of row r: z, water d1 (after evaporation, top-up and
+//         rain), stored depth, stored fluxes      -- loaded at row k+2
+//   B(r)  sediment after transport               -- computed at row k+1
+//   C(r)  new fluxes (flow pass)                 -- computed at row k+1
+//   D(k)  water pass + erosion of row k, stores  -- computed at row k
+// Left/right neighbours come from other lanes (ds_bpermute) with clamped
+// source lanes, so the grid's clamp-to-edge reads hold at the left and right
+// edges; above and below, loads use clamped rows, and the computed rows
+// B/C(-1) and B/C(ny) are replaced by those of row 0 and ny-1.
 
-template <class S, int TW, int TH>
-struct FusedLds
+constexpr int STRIP_OUT = 60; // columns written per wavefront
+
+__device__ __forceinline__ float lane_get(float v, int src_x4)
 {
-  static constexpr int W2 = TW + 4, H2 = TH + 4; // tile + 2-cell halo
-  static constexpr int W1 = TW + 2, H1 = TH + 2; // tile + 1-cell halo
-  float z[H2][W2];
-  float d1[H2][W2];
-  float draw[H2][W2]; // stored depth of the previous iteration
-  float fl[H2][W2], fr[H2][W2], ft[H2][W2], fb[H2][W2];
-  float s[H1][W1];    // sediment after transport
-  Fix128 red[4];
-  float  wta;
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src_x4, __float_as_int(v)));
+}
+
+struct RowA
+{
+  float z, d1, dr, fl, fr, ft, fb;
 };
 
-template <class S, int TW, int TH>
-__global__ void __launch_bounds__(256) k_vp_fused(const float *__restrict__ z,
-                                                  const typename S::T *__restrict__ D,
-                                                  const Flux4<S> *__restrict__ F,
-                                                  const typename S::T *__restrict__ Sp,
-                                                  const float *__restrict__ rain,
-                                                  float *__restrict__ zo,
-                                                  typename S::T *__restrict__ Do,
-                                                  Flux4<S> *__restrict__ Fo,
-                                                  typename S::T *__restrict__ So,
-                                                  const Fix128 *__restrict__ prev_set,
-                                                  Fix128 *set,
-                                                  Fix128 *clear_set,
-                                                  int     tiles_x,
-                                                  int     n_tiles,
-                                                  VpConst k,
-                                                  VpDyn   y)
+template <class S, bool URAIN>
+__device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
+                                            const typename S::T *__restrict__ D,
+                                            const Flux4<S> *__restrict__ F,
+                                            const float *__restrict__ rain,
+                                            int   row_off,
+                                            float topup_add,
+                                            float wta,
+                                            const VpConst &k,
+                                            const VpDyn   &y)
 {
-  using L = FusedLds<S, TW, TH>;
-  __shared__ L sm;
-  const int    nx = k.nx, ny = k.ny;
-  const int    tid = threadIdx.x;
+  RowA     a;
+  Flux4<S> f = F[row_off];
+  a.z = z[row_off];
+  a.dr = S::ld(D[row_off]);
+  a.fl = S::ld(f.l);
+  a.fr = S::ld(f.r);
+  a.ft = S::ld(f.t);
+  a.fb = S::ld(f.b);
+  if (URAIN)
+  {
+    // vp_d1 with rain == 1: (wta * 1) / R and rr * 1 are uniform
+    float d = a.dr;
+    if (y.has_prev)
+    {
+      d = a.dr * k.ce;
+      if (y.topup) d = d + topup_add;
+    }
+    if (k.rr != 0.f) d = d + k.rr;
+    a.d1 = d;
+  }
+  else
+    a.d1 = vp_d1(a.dr, rain[row_off], y.has_prev, y.topup, wta, y.rf, k);
+  return a;
+}
 
-  if (tid == 0) sm.wta = (y.has_prev && y.topup) ? slots_wta(prev_set, y.W0) : 0.f;
+template <class S, bool URAIN>
+__global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
+                                                   const typename S::T *__restrict__ D,
+                                                   const Flux4<S> *__restrict__ F,
+                                                   const typename S::T *__restrict__ Sp,
+                                                   const float *__restrict__ rain,
+                                                   float *__restrict__ zo,
+                                                   typename S::T *__restrict__ Do,
+                                                   Flux4<S> *__restrict__ Fo,
+                                                   typename S::T *__restrict__ So,
+                                                   const Fix128 *__restrict__ prev_set,
+                                                   Fix128 *set,
+                                                   Fix128 *clear_set,
+                                                   int     strips,
+                                                   int     chunk_rows,
+                                                   int     n_items,
+                                                   VpConst k,
+                                                   VpDyn   y)
+{
+  const int nx = k.nx, ny = k.ny;
+  const int lane = threadIdx.x & 63;
   if (k.maintain) clear_slots(clear_set);
 
-  // XCD-aware order: consecutive workgroups go to different XCDs, so each
-  // XCD is given a contiguous run of tiles (shared halos stay in its L2).
+  // XCD-aware work order: workgroup b runs on XCD b % 8; give each XCD a
+  // contiguous range of (chunk, strip) items so halo reuse stays in its L2.
   const int G = gridDim.x;
-  const int xcd = blockIdx.x & 7, lane_blk = blockIdx.x >> 3;
-  const int per_xcd = G >> 3;
-  const int lblk = (G & 7) ? blockIdx.x : xcd * per_xcd + lane_blk;
+  const int b = (G & 7) ? blockIdx.x : (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  const int item = b * 4 + (threadIdx.x >> 6);
+  if (item >= n_items) return;
+  const int strip = item % strips, chunk = item / strips;
+
+  float wta = 0.f, topup_add = 0.f;
+  if (y.has_prev && y.topup)
+  {
+    wta = slots_wta(prev_set, y.W0);
+    topup_add = (wta * 1.f) / y.rf;
+  }
+
+  const int x0 = strip * STRIP_OUT - 2; // grid column of lane 0
+  const int gx = x0 + lane;
+  const int gi = clampi(gx, nx);
+  const int srcl = min(max(clampi(gi - 1, nx) - x0, 0), 63) << 2;
+  const int srcr = min(max(clampi(gi + 1, nx) - x0, 0), 63) << 2;
+  const bool writer = lane >= 2 && lane < 62 && gx < nx;
+
+  const int j0 = chunk * chunk_rows;
+  const int j1 = min(j0 + chunk_rows, ny);
+
+  auto load = [&](int r) {
+    return stream_load<S, URAIN>(z, D, F, rain, clampi(r, ny) * nx + gi, topup_add, wta, k, y);
+  };
+  // B(r): transport of row r from A(r-1), A(r), A(r+1)
+  auto transport = [&](const RowA &am, const RowA &a, const RowA &ap, int r) {
+    const int gj = clampi(r, ny);
+    if (!y.has_prev) return S::ld(Sp[gj * nx + gi]);
+    float frl = lane_get(a.fr, srcl), flr = lane_get(a.fl, srcr);
+    float u, v;
+    vp_velocity(a.dr, frl, am.ft, flr, ap.fb, a.fl, a.fr, a.ft, a.fb, u, v, k);
+    return vp_transport<S>(Sp, gi, gj, u, v, k);
+  };
+  // C(r): flow of row r from A(r-1), A(r), A(r+1); rounded to storage
+  auto flow = [&](const RowA &am, const RowA &a, const RowA &ap, float c[4]) {
+    float zl = lane_get(a.z, srcl), dl = lane_get(a.d1, srcl);
+    float zr = lane_get(a.z, srcr), dr = lane_get(a.d1, srcr);
+    float l = a.fl, r = a.fr, t = a.ft, bb = a.fb;
+    vp_flow(a.z, a.d1, zl, dl, zr, dr, ap.z, ap.d1, am.z, am.d1, l, r, t, bb, k);
+    c[0] = rnd<S>(l);
+    c[1] = rnd<S>(r);
+    c[2] = rnd<S>(t);
+    c[3] = rnd<S>(bb);
+  };
+
+  // pre-roll: A(j0-2..j0+1), B/C(j0-1), B/C(j0)
+  RowA  am2 = load(j0 - 2), am1 = load(j0 - 1), a0 = load(j0), ap1 = load(j0 + 1);
+  float s_m1 = transport(am2, am1, a0, j0 - 1);
+  float c_m1[4];
+  flow(am2, am1, a0, c_m1);
+  float s_0 = transport(am1, a0, ap1, j0);
+  float c_0[4];
+  flow(am1, a0, ap1, c_0);
 
   Fix128 acc{0, 0};
-  __syncthreads();
-  const float wta = sm.wta;
-
-  for (int tile = lblk; tile < n_tiles; tile += G)
+  for (int r = j0; r < j1; r++)
   {
-    const int ti0 = (tile % tiles_x) * TW, tj0 = (tile / tiles_x) * TH;
+    RowA  ap2 = load(r + 2);
+    float s_p1 = transport(a0, ap1, ap2, r + 1);
+    float c_p1[4];
+    flow(a0, ap1, ap2, c_p1);
 
-    // Phase A: stage z, water and fluxes of tile + 2 (clamped to the edge).
-    for (int e = tid; e < L::W2 * L::H2; e += blockDim.x)
+    // row r: water pass and erosion
+    const bool top = r == 0, bot = r == ny - 1;
+    float      s_lo = top ? s_0 : s_m1, s_hi = bot ? s_0 : s_p1;
+    float      ftb = top ? c_0[2] : c_m1[2], fbt = bot ? c_0[3] : c_p1[3];
+    float      frl = lane_get(c_0[1], srcl), flr = lane_get(c_0[0], srcr);
+    float      d2 = rnd<S>(vp_depth(a0.d1, frl, ftb, flr, fbt, c_0[0], c_0[1], c_0[2], c_0[3], k));
+    float      u, v;
+    vp_velocity(d2, frl, ftb, flr, fbt, c_0[0], c_0[1], c_0[2], c_0[3], u, v, k);
+    float zl = lane_get(a0.z, srcl), zr = lane_get(a0.z, srcr);
+    float sl = lane_get(s_0, srcl), sr = lane_get(s_0, srcr);
+    float zn, sn;
+    vp_erosion(a0.z, s_0, zr, zl, ap1.z, am1.z, sr, sl, s_hi, s_lo, d2, u, v, zn, sn, k);
+    if (writer)
     {
-      int      lx = e % L::W2, ly = e / L::W2;
-      int      gi = clampi(ti0 + lx - 2, nx), gj = clampi(tj0 + ly - 2, ny);
-      size_t   c = (size_t)gj * nx + gi;
-      float    dr = S::ld(D[c]);
-      Flux4<S> f = F[c];
-      float    r = rain ? rain[c] : 1.f;
-      sm.z[ly][lx] = z[c];
-      sm.draw[ly][lx] = dr;
-      sm.d1[ly][lx] = vp_d1(dr, r, y.has_prev, y.topup, wta, y.rf, k);
-      sm.fl[ly][lx] = S::ld(f.l);
-      sm.fr[ly][lx] = S::ld(f.r);
-      sm.ft[ly][lx] = S::ld(f.t);
-      sm.fb[ly][lx] = S::ld(f.b);
-    }
-    __syncthreads();
-
-    // Phase B: sediment transport on tile + 1 (reads the stored sediment
-    // plane at the backtrace position, anywhere in the grid).
-    for (int e = tid; e < L::W1 * L::H1; e += blockDim.x)
-    {
-      int lx = e % L::W1, ly = e / L::W1;
-      int gi = clampi(ti0 + lx - 1, nx), gj = clampi(tj0 + ly - 1, ny);
-      float sv;
-      if (y.has_prev)
-      {
-        // neighbours in the staged planes; indices clamped like the grid
-        int   cx = gi - ti0 + 2, cy = gj - tj0 + 2;
-        int   xl = clampi(gi - 1, nx) - ti0 + 2, xr = clampi(gi + 1, nx) - ti0 + 2;
-        int   yb = clampi(gj - 1, ny) - tj0 + 2, yt = clampi(gj + 1, ny) - tj0 + 2;
-        float u, v;
-        vp_velocity(sm.draw[cy][cx], sm.fr[cy][xl], sm.ft[yb][cx], sm.fl[cy][xr],
-                    sm.fb[yt][cx], sm.fl[cy][cx], sm.fr[cy][cx], sm.ft[cy][cx],
-                    sm.fb[cy][cx], u, v, k);
-        sv = vp_transport<S>(Sp, gi, gj, u, v, k);
-      }
-      else
-        sv = S::ld(Sp[(size_t)gj * nx + gi]);
-      sm.s[ly][lx] = sv;
-    }
-
-    // Phase C: flow on tile + 1 from the staged z, d1 and own fluxes.
-    constexpr int NF = (L::W1 * L::H1 + 255) / 256; // cells per lane
-    float         fo[NF][4];
-    int   nflow = 0;
-    for (int e = tid; e < L::W1 * L::H1; e += blockDim.x, nflow++)
-    {
-      int lx = e % L::W1 + 1, ly = e / L::W1 + 1; // staged-plane coordinates
-      int gi = clampi(ti0 + lx - 2, nx), gj = clampi(tj0 + ly - 2, ny);
-      int cx = gi - ti0 + 2, cy = gj - tj0 + 2;
-      int xl = clampi(gi - 1, nx) - ti0 + 2, xr = clampi(gi + 1, nx) - ti0 + 2;
-      int yb = clampi(gj - 1, ny) - tj0 + 2, yt = clampi(gj + 1, ny) - tj0 + 2;
-      float l = sm.fl[cy][cx], r = sm.fr[cy][cx], t = sm.ft[cy][cx], b = sm.fb[cy][cx];
-      vp_flow(sm.z[cy][cx], sm.d1[cy][cx], sm.z[cy][xl], sm.d1[cy][xl], sm.z[cy][xr],
-              sm.d1[cy][xr], sm.z[yt][cx], sm.d1[yt][cx], sm.z[yb][cx], sm.d1[yb][cx], l, r,
-              t, b, k);
-      fo[nflow][0] = rnd<S>(l);
-      fo[nflow][1] = rnd<S>(r);
-      fo[nflow][2] = rnd<S>(t);
-      fo[nflow][3] = rnd<S>(b);
-    }
-    __syncthreads();
-    nflow = 0;
-    for (int e = tid; e < L::W1 * L::H1; e += blockDim.x, nflow++)
-    {
-      int lx = e % L::W1 + 1, ly = e / L::W1 + 1;
-      sm.fl[ly][lx] = fo[nflow][0];
-      sm.fr[ly][lx] = fo[nflow][1];
-      sm.ft[ly][lx] = fo[nflow][2];
-      sm.fb[ly][lx] = fo[nflow][3];
-    }
-    __syncthreads();
-
-    // Phase D: water and erosion on the tile; write the new state.
-    for (int e = tid; e < TW * TH; e += blockDim.x)
-    {
-      int lx = e % TW, ly = e / TW;
-      int gi = ti0 + lx, gj = tj0 + ly;
-      if (gi >= nx || gj >= ny) continue;
-      int   cx = lx + 2, cy = ly + 2;
-      int   xl = clampi(gi - 1, nx) - ti0 + 2, xr = clampi(gi + 1, nx) - ti0 + 2;
-      int   yb = clampi(gj - 1, ny) - tj0 + 2, yt = clampi(gj + 1, ny) - tj0 + 2;
-      float fl = sm.fl[cy][cx], fr = sm.fr[cy][cx], ft = sm.ft[cy][cx], fb = sm.fb[cy][cx];
-      float frl = sm.fr[cy][xl], flr = sm.fl[cy][xr], ftb = sm.ft[yb][cx], fbt = sm.fb[yt][cx];
-      float d2 = rnd<S>(vp_depth(sm.d1[cy][cx], frl, ftb, flr, fbt, fl, fr, ft, fb, k));
-      float u, v;
-      vp_velocity(d2, frl, ftb, flr, fbt, fl, fr, ft, fb, u, v, k);
-      // sediment plane is staged on tile + 1: offset 1
-      float zn, sn;
-      vp_erosion(sm.z[cy][cx], sm.s[cy - 1][cx - 1], sm.z[cy][xr], sm.z[cy][xl],
-                 sm.z[yt][cx], sm.z[yb][cx], sm.s[cy - 1][xr - 1], sm.s[cy - 1][xl - 1],
-                 sm.s[yt - 1][cx - 1], sm.s[yb - 1][cx - 1], d2, u, v, zn, sn, k);
-      size_t c = (size_t)gj * nx + gi;
+      const int c = r * nx + gx;
       zo[c] = zn;
       So[c] = S::st(sn);
       Do[c] = S::st(d2);
       Flux4<S> o;
-      o.l = S::st(fl);
-      o.r = S::st(fr);
-      o.t = S::st(ft);
-      o.b = S::st(fb);
+      o.l = S::st(c_0[0]);
+      o.r = S::st(c_0[1]);
+      o.t = S::st(c_0[2]);
+      o.b = S::st(c_0[3]);
       Fo[c] = o;
       if (k.maintain) fix_add(acc, fix_term(d2 * k.ce));
     }
-    __syncthreads();
+
+    // shift the window
+    am1 = a0;
+    a0 = ap1;
+    ap1 = ap2;
+    s_m1 = s_0;
+    s_0 = s_p1;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+    {
+      c_m1[q] = c_0[q];
+      c_0[q] = c_p1[q];
+    }
   }
-  if (k.maintain) block_fix_atomic(acc, sm.red, set);
+
+  if (k.maintain)
+  {
+    for (int o = 32; o > 0; o >>= 1)
+    {
+      Fix128 other;
+      other.lo = __shfl_xor(acc.lo, o, 64);
+      other.hi = __shfl_xor(acc.hi, o, 64);
+      fix_add(acc, other);
+    }
+    if (lane == 0)
+    {
+      Fix128            *slot = set + (item % NSLOT);
+      unsigned long long old = atomicAdd((unsigned long long *)&slot->lo, (unsigned long long)acc.lo);
+      unsigned long long carry = (old + acc.lo < old) ? 1ull : 0ull;
+      atomicAdd((unsigned long long *)&slot->hi, (unsigned long long)acc.hi + carry);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- export
@@ -507,7 +551,7 @@ struct hmg_session
   long              iter = 0;
   double            W0 = 0.0, R = 0.0;
   float             rf = 0.f;
-  int               tile_w = 64, tile_h = 16;
+  int               chunk_rows = 64;
 };
 
 namespace hmg
@@ -626,24 +670,33 @@ static int session_step_plain(hmg_session *s)
   return HMG_OK;
 }
 
-template <class S, int TW, int TH>
+template <class S>
 static int session_step_fused(hmg_session *s)
 {
   using T = typename S::T;
   const int   nx = s->nx, ny = s->ny;
   hipStream_t st = s->ctx->stream;
   const int   a = s->cur, b = s->cur ^ 1;
-  const int   tiles_x = (nx + TW - 1) / TW, tiles_y = (ny + TH - 1) / TH;
-  const int   n_tiles = tiles_x * tiles_y;
-  int         G = s->ctx->num_cus * 4;
-  if (G > n_tiles) G = n_tiles;
-  VpDyn y = make_dyn(s);
-  k_vp_fused<S, TW, TH><<<G, 256, 0, st>>>(
-      s->z[a].as<float>(), (const T *)s->D[a].ptr, (const Flux4<S> *)s->F[a].ptr,
-      (const T *)s->Sp[a].ptr, s->rain.as<float>(), s->z[b].as<float>(), (T *)s->D[b].ptr,
-      (Flux4<S> *)s->F[b].ptr, (T *)s->Sp[b].ptr, slot_set(s, s->iter - 1),
-      slot_set(s, s->iter), slot_set(s, s->iter + 1), tiles_x, n_tiles, s->k, y);
-  HMG_TRY(check_launch("vp_fused"));
+  const int   strips = (nx + STRIP_OUT - 1) / STRIP_OUT;
+  const int   ch = s->chunk_rows;
+  const int   n_items = strips * ((ny + ch - 1) / ch);
+  const int   G = (n_items + 3) / 4;
+  VpDyn       y = make_dyn(s);
+  const T    *D = (const T *)s->D[a].ptr;
+  const T    *Sp = (const T *)s->Sp[a].ptr;
+  if (s->rain.ptr)
+    k_vp_stream<S, false><<<G, 256, 0, st>>>(
+        s->z[a].as<float>(), D, (const Flux4<S> *)s->F[a].ptr, Sp, s->rain.as<float>(),
+        s->z[b].as<float>(), (T *)s->D[b].ptr, (Flux4<S> *)s->F[b].ptr, (T *)s->Sp[b].ptr,
+        slot_set(s, s->iter - 1), slot_set(s, s->iter), slot_set(s, s->iter + 1), strips, ch,
+        n_items, s->k, y);
+  else
+    k_vp_stream<S, true><<<G, 256, 0, st>>>(
+        s->z[a].as<float>(), D, (const Flux4<S> *)s->F[a].ptr, Sp, nullptr,
+        s->z[b].as<float>(), (T *)s->D[b].ptr, (Flux4<S> *)s->F[b].ptr, (T *)s->Sp[b].ptr,
+        slot_set(s, s->iter - 1), slot_set(s, s->iter), slot_set(s, s->iter + 1), strips, ch,
+        n_items, s->k, y);
+  HMG_TRY(check_launch("vp_stream"));
   s->cur = b;
   s->iter++;
   return HMG_OK;
@@ -653,7 +706,7 @@ template <class S>
 static int session_step(hmg_session *s)
 {
   if (s->p.mode == HMG_VP_PLAIN) return session_step_plain<S>(s);
-  return session_step_fused<S, 64, 16>(s);
+  return session_step_fused<S>(s);
 }
 
 template <class S>
@@ -910,6 +963,17 @@ int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d
     }
     if (dst[q]) *dst[q] = fix_to_double(a.lo, a.hi);
   }
+  return HMG_OK;
+}
+
+int hmg_session_set_tuning(hmg_session *s, int chunk_rows)
+{
+  clear_error();
+  if (!s) return set_error(HMG_ERR_INVALID, "null session");
+  if (chunk_rows < 1 || chunk_rows > 65536)
+    return set_error(HMG_ERR_INVALID, "chunk_rows must be in [1, 65536]");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  s->chunk_rows = chunk_rows;
   return HMG_OK;
 }
 
