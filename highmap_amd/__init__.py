@@ -130,7 +130,7 @@ def lib():
             "hmg_session_sums": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)], i32),
             "hmg_session_z_dev": ([vp], vp),
-            "hmg_session_set_tuning": ([vp, i32], i32),
+            "hmg_session_set_tuning": ([vp, i32, i32], i32),
             "hmg_session_sync": ([vp], i32),
             "hmg_session_destroy": ([vp], i32),
             "hmg_session_bytes": ([i32, i32, i32], sz),
@@ -341,8 +341,8 @@ class Session:
         _check(lib().hmg_session_export(self.handle, *[_ptr(a) for a in out]))
         return tuple(out)
 
-    def set_tuning(self, chunk_rows):
-        _check(lib().hmg_session_set_tuning(self.handle, int(chunk_rows)))
+    def set_tuning(self, chunk_rows, prefetch=1):
+        _check(lib().hmg_session_set_tuning(self.handle, int(chunk_rows), int(prefetch)))
         return self
 
     def z_dev_ptr(self):

@@ -306,7 +306,9 @@ __device__ __forceinline__ float hm_hypot(float x, float y)
 
 // hydraulic_vpipes_erosion_pass (hydraulic_vpipes.cl:149-183) for one cell.
 // zr/zl/zt/zb and sr/sl/st/sb are the neighbour values right (i+1), left
-// (i-1), top (j+1), bottom (j-1).
+// (i-1), top (j+1), bottom (j-1).  KEXP1: k_discharge_exp is known to be 1,
+// so pow(discharge, 1) = discharge is not evaluated at all.
+template <bool KEXP1 = false>
 __device__ __forceinline__ void vp_erosion(float        zc,
                                            float        sc,
                                            float        zr,
@@ -332,7 +334,7 @@ __device__ __forceinline__ void vp_erosion(float        zc,
   float speed = hm_hypot(u, v);
   float depth = fminf(d2 / k.dmax, k.ratio);
   float discharge = depth * speed;
-  float pw = k.kexp == 1.f ? discharge : powf(discharge, k.kexp);
+  float pw = (KEXP1 || k.kexp == 1.f) ? discharge : powf(discharge, k.kexp);
   float capa = k.kc * pw * salpha;
   if (capa > sc)
   {
@@ -367,6 +369,48 @@ __device__ __forceinline__ void mirror_axis(float p,
   int kq = (int)q0;
   i0 = min(max(kq, 0), w - 1);
   i1 = min(max(kq + 1, 0), w - 1);
+}
+
+// The transport sample split in two: `issue` computes the weights and loads
+// the four texels, `finish` forms the weighted sum, so that other work can
+// sit between the loads and their use.  Same operations as vp_transport.
+struct TexelFetch
+{
+  float w00, w10, w01, w11;
+  float s00, s10, s01, s11;
+};
+
+template <class S>
+__device__ __forceinline__ TexelFetch vp_transport_issue(const typename S::T *sp,
+                                                         int                  i,
+                                                         int                  j,
+                                                         float                u,
+                                                         float                v,
+                                                         const VpConst       &k)
+{
+  float      px = (float)i - k.dt * u;
+  float      py = (float)j - k.dt * v;
+  int        i0, i1, j0, j1;
+  float      a, b;
+  TexelFetch t;
+  mirror_axis(px, k.nx, k.inv_nx, k.pow2x, i0, i1, a);
+  mirror_axis(py, k.ny, k.inv_ny, k.pow2y, j0, j1, b);
+  t.w00 = (1.f - a) * (1.f - b);
+  t.w10 = a * (1.f - b);
+  t.w01 = (1.f - a) * b;
+  t.w11 = a * b;
+  const typename S::T *r0 = sp + j0 * k.nx;
+  const typename S::T *r1 = sp + j1 * k.nx;
+  t.s00 = S::ld(r0[i0]);
+  t.s10 = S::ld(r0[i1]);
+  t.s01 = S::ld(r1[i0]);
+  t.s11 = S::ld(r1[i1]);
+  return t;
+}
+
+__device__ __forceinline__ float vp_transport_finish(const TexelFetch &t)
+{
+  return t.w00 * t.s00 + t.w10 * t.s10 + t.w01 * t.s01 + t.w11 * t.s11;
 }
 
 // Sediment transport (hydraulic_vpipes.cl:204-210): bilinear sample of the

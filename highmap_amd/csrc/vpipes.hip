@@ -313,7 +313,7 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
   return a;
 }
 
-template <class S, bool URAIN>
+template <class S, bool URAIN, bool PF, bool KEXP1>
 __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
                                                    const typename S::T *__restrict__ D,
                                                    const Flux4<S> *__restrict__ F,
@@ -364,14 +364,29 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
   auto load = [&](int r) {
     return stream_load<S, URAIN>(z, D, F, rain, clampi(r, ny) * nx + gi, topup_add, wta, k, y);
   };
-  // B(r): transport of row r from A(r-1), A(r), A(r+1)
-  auto transport = [&](const RowA &am, const RowA &a, const RowA &ap, int r) {
-    const int gj = clampi(r, ny);
-    if (!y.has_prev) return S::ld(Sp[gj * nx + gi]);
+  // B(r): transport of row r from A(r-1), A(r), A(r+1), in two halves
+  auto transport_issue = [&](const RowA &am, const RowA &a, const RowA &ap, int r) {
+    const int  gj = clampi(r, ny);
+    TexelFetch t;
+    if (!y.has_prev)
+    {
+      t.w00 = 1.f;
+      t.w10 = t.w01 = t.w11 = 0.f;
+      t.s00 = S::ld(Sp[gj * nx + gi]);
+      t.s10 = t.s01 = t.s11 = 0.f;
+      return t;
+    }
     float frl = lane_get(a.fr, srcl), flr = lane_get(a.fl, srcr);
     float u, v;
     vp_velocity(a.dr, frl, am.ft, flr, ap.fb, a.fl, a.fr, a.ft, a.fb, u, v, k);
-    return vp_transport<S>(Sp, gi, gj, u, v, k);
+    return vp_transport_issue<S>(Sp, gi, gj, u, v, k);
+  };
+  auto transport_finish = [&](const TexelFetch &t) {
+    // first iteration of a session: the stored sediment, untransported
+    return y.has_prev ? vp_transport_finish(t) : t.s00;
+  };
+  auto transport = [&](const RowA &am, const RowA &a, const RowA &ap, int r) {
+    return transport_finish(transport_issue(am, a, ap, r));
   };
   // C(r): flow of row r from A(r-1), A(r), A(r+1); rounded to storage
   auto flow = [&](const RowA &am, const RowA &a, const RowA &ap, float c[4]) {
@@ -395,12 +410,23 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
   flow(am1, a0, ap1, c_0);
 
   Fix128 acc{0, 0};
+  RowA   apf;
+  if (PF) apf = load(j0 + 2);
   for (int r = j0; r < j1; r++)
   {
-    RowA  ap2 = load(r + 2);
-    float s_p1 = transport(a0, ap1, ap2, r + 1);
-    float c_p1[4];
+    RowA ap2;
+    if (PF)
+    {
+      // rows are loaded one iteration before they are used
+      ap2 = apf;
+      apf = load(r + 3);
+    }
+    else
+      ap2 = load(r + 2);
+    TexelFetch tf = transport_issue(a0, ap1, ap2, r + 1);
+    float      c_p1[4];
     flow(a0, ap1, ap2, c_p1);
+    float s_p1 = transport_finish(tf);
 
     // row r: water pass and erosion
     const bool top = r == 0, bot = r == ny - 1;
@@ -413,7 +439,7 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
     float zl = lane_get(a0.z, srcl), zr = lane_get(a0.z, srcr);
     float sl = lane_get(s_0, srcl), sr = lane_get(s_0, srcr);
     float zn, sn;
-    vp_erosion(a0.z, s_0, zr, zl, ap1.z, am1.z, sr, sl, s_hi, s_lo, d2, u, v, zn, sn, k);
+    vp_erosion<KEXP1>(a0.z, s_0, zr, zl, ap1.z, am1.z, sr, sl, s_hi, s_lo, d2, u, v, zn, sn, k);
     if (writer)
     {
       const int c = r * nx + gx;
@@ -552,6 +578,7 @@ struct hmg_session
   double            W0 = 0.0, R = 0.0;
   float             rf = 0.f;
   int               chunk_rows = 64;
+  int               prefetch = 1;
 };
 
 namespace hmg
@@ -684,18 +711,40 @@ static int session_step_fused(hmg_session *s)
   VpDyn       y = make_dyn(s);
   const T    *D = (const T *)s->D[a].ptr;
   const T    *Sp = (const T *)s->Sp[a].ptr;
-  if (s->rain.ptr)
-    k_vp_stream<S, false><<<G, 256, 0, st>>>(
-        s->z[a].as<float>(), D, (const Flux4<S> *)s->F[a].ptr, Sp, s->rain.as<float>(),
-        s->z[b].as<float>(), (T *)s->D[b].ptr, (Flux4<S> *)s->F[b].ptr, (T *)s->Sp[b].ptr,
-        slot_set(s, s->iter - 1), slot_set(s, s->iter), slot_set(s, s->iter + 1), strips, ch,
-        n_items, s->k, y);
+  const float *rain = s->rain.as<float>();
+  float       *zo = s->z[b].as<float>();
+  T           *Do = (T *)s->D[b].ptr, *So = (T *)s->Sp[b].ptr;
+  Flux4<S>    *Fo = (Flux4<S> *)s->F[b].ptr;
+  const Flux4<S> *F = (const Flux4<S> *)s->F[a].ptr;
+  Fix128      *pset = slot_set(s, s->iter - 1), *aset = slot_set(s, s->iter);
+  Fix128      *cset = slot_set(s, s->iter + 1);
+  const float *z = s->z[a].as<float>();
+#define HMG_STREAM(UR, PFV, KE)                                                \
+  k_vp_stream<S, UR, PFV, KE><<<G, 256, 0, st>>>(z, D, F, Sp, rain, zo, Do, Fo, So, pset, \
+                                                 aset, cset, strips, ch, n_items, s->k, y)
+  const bool ke1 = s->k.kexp == 1.f;
+  if (rain)
+  {
+    if (ke1)
+      HMG_STREAM(false, true, true);
+    else
+      HMG_STREAM(false, true, false);
+  }
+  else if (!s->prefetch)
+  {
+    if (ke1)
+      HMG_STREAM(true, false, true);
+    else
+      HMG_STREAM(true, false, false);
+  }
   else
-    k_vp_stream<S, true><<<G, 256, 0, st>>>(
-        s->z[a].as<float>(), D, (const Flux4<S> *)s->F[a].ptr, Sp, nullptr,
-        s->z[b].as<float>(), (T *)s->D[b].ptr, (Flux4<S> *)s->F[b].ptr, (T *)s->Sp[b].ptr,
-        slot_set(s, s->iter - 1), slot_set(s, s->iter), slot_set(s, s->iter + 1), strips, ch,
-        n_items, s->k, y);
+  {
+    if (ke1)
+      HMG_STREAM(true, true, true);
+    else
+      HMG_STREAM(true, true, false);
+  }
+#undef HMG_STREAM
   HMG_TRY(check_launch("vp_stream"));
   s->cur = b;
   s->iter++;
@@ -966,14 +1015,16 @@ int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d
   return HMG_OK;
 }
 
-int hmg_session_set_tuning(hmg_session *s, int chunk_rows)
+int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch)
 {
   clear_error();
   if (!s) return set_error(HMG_ERR_INVALID, "null session");
   if (chunk_rows < 1 || chunk_rows > 65536)
     return set_error(HMG_ERR_INVALID, "chunk_rows must be in [1, 65536]");
+  if (prefetch != 0 && prefetch != 1) return set_error(HMG_ERR_INVALID, "prefetch must be 0 or 1");
   std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
   s->chunk_rows = chunk_rows;
+  s->prefetch = prefetch;
   return HMG_OK;
 }
 

@@ -255,8 +255,9 @@ int hmg_session_export(hmg_session *s,
 /* Waits, then returns sum(z), sum(s), sum(d) in FP64 (mass check). */
 int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d);
 /* Launch shape of the fused kernel: rows each wavefront streams through
- * (default 64).  Results do not depend on it. */
-int hmg_session_set_tuning(hmg_session *s, int chunk_rows);
+ * (default 64) and whether each row's loads are issued one row ahead
+ * (default 1).  Results do not depend on either. */
+int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch);
 /* Device pointer of the session's current z plane (FP32, nx*ny). */
 float *hmg_session_z_dev(hmg_session *s);
 int    hmg_session_sync(hmg_session *s);

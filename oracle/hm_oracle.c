@@ -370,6 +370,29 @@ double orc_fixsum_f32(const float *a, size_t n)
   return orc_fix_to_double(s);
 }
 
+/* The 128-bit sum itself, as two's-complement limbs (test helper). */
+void orc_fixsum_parts(const float *a, size_t n, uint64_t *lo, int64_t *hi)
+{
+  i128 s = 0;
+  for (size_t k = 0; k < n; k++) s += orc_fix_term(a[k]);
+  *lo = (uint64_t)(u128)s;
+  *hi = (int64_t)(uint64_t)((u128)s >> 64);
+}
+
+/* Number of xorshift32 steps from `start` back to `start` (test helper:
+ * 2^32 - 1 for a full-period shift triple). */
+uint64_t orc_xorshift_period(uint32_t start)
+{
+  uint32_t x = start;
+  uint64_t n = 0;
+  do
+  {
+    x = orc_xorshift32(x);
+    n++;
+  } while (x != start && n < ((uint64_t)1 << 33));
+  return n;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Thermal and vpipes, instantiated for float and double.                    */
 /* ------------------------------------------------------------------------ */

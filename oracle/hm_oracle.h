@@ -107,6 +107,8 @@ void orc_extrapolate_borders_f64(double *z, int nx, int ny);
 
 /* --- fixed-point water volume (order-independent, exact per term) --- */
 double orc_fixsum_f32(const float *a, size_t n);
+void   orc_fixsum_parts(const float *a, size_t n, uint64_t *lo, int64_t *hi);
+uint64_t orc_xorshift_period(uint32_t start);
 
 /* --- vpipes --- */
 /* Initial state: d = water_height * rain, s = fl = fr = ft = fb = u = v = 0.

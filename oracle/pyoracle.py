@@ -103,6 +103,10 @@ def lib():
     L.orc_extrapolate_borders_f64.argtypes = [F64P, C.c_int, C.c_int]
     L.orc_fixsum_f32.argtypes = [F32P, C.c_size_t]
     L.orc_fixsum_f32.restype = C.c_double
+    L.orc_fixsum_parts.argtypes = [F32P, C.c_size_t, C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_int64)]
+    L.orc_xorshift_period.argtypes = [u32]
+    L.orc_xorshift_period.restype = C.c_uint64
     pp = C.POINTER(VpParams)
     dp = C.POINTER(C.c_double)
     L.orc_vpipes_init_f32.argtypes = [F32P] * 8 + [C.c_int, C.c_int, pp, C.c_void_p, dp, dp]
@@ -238,3 +242,12 @@ def hydraulic_vpipes(z, params=None, rain=None, iterations=50, dtype=np.float32)
     """hmap::gpu::hydraulic_vpipes restated; returns (z, d, s, u, v)."""
     st = VpState(z, params or vp_params(), rain, dtype).run(iterations)
     return st.z, st.d, st.s, st.u, st.v
+
+
+def fixsum_parts(a):
+    """Exact 128-bit fixed-point sum of round(a * 2^64) as a Python int."""
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    lo = C.c_uint64()
+    hi = C.c_int64()
+    lib().orc_fixsum_parts(a, a.size, C.byref(lo), C.byref(hi))
+    return (hi.value << 64) + lo.value
