@@ -1,0 +1,244 @@
+// hmap_gpu.cpp -- the hmap::gpu drop-in functions over libhmg's C ABI.
+//
+// Each function keeps the reference's host-side contract: host arrays in,
+// results written back into the same arrays, optional outputs assigned when
+// their pointer is not null.  The device work is libhmg's; nothing here
+// computes the hot path on the CPU.
+#include "hmap_gpu.hpp"
+
+#include <algorithm>
+#include <mutex>
+#include <numeric>
+#include <string>
+
+#include "hmg.h"
+
+namespace hmap
+{
+
+float Array::sum() const
+{
+  // Array::sum (HighMap/src/array/methods.cpp:428-431)
+  return std::accumulate(vector.begin(), vector.end(), 0.f);
+}
+
+float Array::min() const
+{
+  return vector.empty() ? 0.f : *std::min_element(vector.begin(), vector.end());
+}
+
+float Array::max() const
+{
+  return vector.empty() ? 0.f : *std::max_element(vector.begin(), vector.end());
+}
+
+void remap(Array &array, float vmin, float vmax)
+{
+  if (array.vector.empty()) return;
+  const float mn = array.min(), mx = array.max();
+  if (mn == mx)
+  {
+    std::fill(array.vector.begin(), array.vector.end(), vmin);
+    return;
+  }
+  const float scale = (vmax - vmin) / (mx - mn);
+  const float offset = vmin - mn * scale;
+  for (float &x : array.vector) x = x * scale + offset;
+}
+
+namespace gpu
+{
+
+namespace
+{
+std::mutex g_mu;
+hmg_ctx   *g_ctx = nullptr;
+
+hmg_ctx *ctx()
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ctx && hmg_init(0, &g_ctx) != HMG_OK)
+    throw std::runtime_error(std::string("hmap::gpu: no GPU: ") + hmg_last_error());
+  return g_ctx;
+}
+
+void check(int rc, const char *what)
+{
+  if (rc != HMG_OK)
+    throw std::runtime_error(std::string("hmap::gpu::") + what + ": " + hmg_last_error());
+}
+
+void check_shape(const Array &a, glm::ivec2 s, const char *what)
+{
+  if (a.shape.x != s.x || a.shape.y != s.y || a.vector.size() != (size_t)s.x * (size_t)s.y)
+    throw std::invalid_argument(std::string("hmap::gpu: shape mismatch for ") + what);
+}
+} // namespace
+
+bool init_opencl()
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_ctx) return true;
+  return hmg_init(0, &g_ctx) == HMG_OK;
+}
+
+Array noise_fbm(NoiseType     noise_type,
+                glm::ivec2    shape,
+                glm::vec2     kw,
+                std::uint32_t seed,
+                int           octaves,
+                float         weight,
+                float         persistence,
+                float         lacunarity,
+                const Array  *p_ctrl_param,
+                const Array  *p_noise_x,
+                const Array  *p_noise_y,
+                const Array * /* p_stretching: ignored, as in the reference */,
+                glm::vec4     bbox,
+                glm::ivec2    period)
+{
+  Array out(shape);
+  if (p_ctrl_param) check_shape(*p_ctrl_param, shape, "ctrl_param");
+  if (p_noise_x) check_shape(*p_noise_x, shape, "noise_x");
+  if (p_noise_y) check_shape(*p_noise_y, shape, "noise_y");
+  const float bb[4] = {bbox.x, bbox.y, bbox.z, bbox.w};
+  const int   per[2] = {period.x, period.y};
+  check(hmg_noise_fbm(ctx(),
+                      out.vector.data(),
+                      shape.x,
+                      shape.y,
+                      (int)noise_type,
+                      kw.x,
+                      kw.y,
+                      seed,
+                      octaves,
+                      weight,
+                      persistence,
+                      lacunarity,
+                      p_ctrl_param ? p_ctrl_param->vector.data() : nullptr,
+                      p_noise_x ? p_noise_x->vector.data() : nullptr,
+                      p_noise_y ? p_noise_y->vector.data() : nullptr,
+                      bb,
+                      per),
+        "noise_fbm");
+  return out;
+}
+
+void thermal(Array       &z,
+             const Array &talus,
+             int          iterations,
+             Array       *p_bedrock,
+             Array       *p_deposition_map)
+{
+  check_shape(talus, z.shape, "talus");
+  if (p_bedrock) check_shape(*p_bedrock, z.shape, "bedrock");
+  if (p_deposition_map) *p_deposition_map = Array(z.shape);
+  check(hmg_thermal(ctx(),
+                    z.vector.data(),
+                    z.shape.x,
+                    z.shape.y,
+                    talus.vector.data(),
+                    0.f,
+                    iterations,
+                    p_bedrock ? p_bedrock->vector.data() : nullptr,
+                    p_deposition_map ? p_deposition_map->vector.data() : nullptr,
+                    nullptr),
+        "thermal");
+}
+
+void thermal(Array       &z,
+             const Array *p_mask,
+             const Array &talus,
+             int          iterations,
+             Array       *p_bedrock,
+             Array       *p_deposition_map)
+{
+  check_shape(talus, z.shape, "talus");
+  if (p_mask) check_shape(*p_mask, z.shape, "mask");
+  if (p_bedrock) check_shape(*p_bedrock, z.shape, "bedrock");
+  if (p_deposition_map) *p_deposition_map = Array(z.shape);
+  check(hmg_thermal(ctx(),
+                    z.vector.data(),
+                    z.shape.x,
+                    z.shape.y,
+                    talus.vector.data(),
+                    0.f,
+                    iterations,
+                    p_bedrock ? p_bedrock->vector.data() : nullptr,
+                    p_deposition_map ? p_deposition_map->vector.data() : nullptr,
+                    p_mask ? p_mask->vector.data() : nullptr),
+        "thermal");
+}
+
+void thermal(Array &z, float talus, int iterations, Array *p_bedrock, Array *p_deposition_map)
+{
+  // the reference materialises Array(shape, talus) (thermal_gpu.cpp:94);
+  // the scalar is passed through instead, with identical results
+  if (p_bedrock) check_shape(*p_bedrock, z.shape, "bedrock");
+  if (p_deposition_map) *p_deposition_map = Array(z.shape);
+  check(hmg_thermal(ctx(),
+                    z.vector.data(),
+                    z.shape.x,
+                    z.shape.y,
+                    nullptr,
+                    talus,
+                    iterations,
+                    p_bedrock ? p_bedrock->vector.data() : nullptr,
+                    p_deposition_map ? p_deposition_map->vector.data() : nullptr,
+                    nullptr),
+        "thermal");
+}
+
+void hydraulic_vpipes(Array &z,
+                      float  water_height,
+                      bool   maintain_water_volume,
+                      float  evap_rate,
+                      int    iterations,
+                      float  dt,
+                      float  k_capacity,
+                      float  k_erode,
+                      float  k_depose,
+                      float  k_discharge_exp,
+                      float  downcutting_max_depth_ratio,
+                      bool   flux_diffusion,
+                      float  flux_diffusion_strength,
+                      Array *p_rain_map,
+                      Array *p_water_depth,
+                      Array *p_sediment,
+                      Array *p_vel_u,
+                      Array *p_vel_v)
+{
+  hmg_vpipes_params p;
+  hmg_vpipes_defaults(&p);
+  p.water_height = water_height;
+  p.maintain_water_volume = maintain_water_volume ? 1 : 0;
+  p.evap_rate = evap_rate;
+  p.iterations = iterations;
+  p.dt = dt;
+  p.k_capacity = k_capacity;
+  p.k_erode = k_erode;
+  p.k_depose = k_depose;
+  p.k_discharge_exp = k_discharge_exp;
+  p.downcutting_max_depth_ratio = downcutting_max_depth_ratio;
+  p.flux_diffusion = flux_diffusion ? 1 : 0;
+  p.flux_diffusion_strength = flux_diffusion_strength;
+  if (p_rain_map) check_shape(*p_rain_map, z.shape, "rain_map");
+  // optional outputs are assigned by value, as hydraulic_vpipes_gpu.cpp:174-177
+  Array *outs[4] = {p_water_depth, p_sediment, p_vel_u, p_vel_v};
+  for (Array *o : outs)
+    if (o) *o = Array(z.shape);
+  check(hmg_vpipes(ctx(),
+                   z.vector.data(),
+                   z.shape.x,
+                   z.shape.y,
+                   &p,
+                   p_rain_map ? p_rain_map->vector.data() : nullptr,
+                   p_water_depth ? p_water_depth->vector.data() : nullptr,
+                   p_sediment ? p_sediment->vector.data() : nullptr,
+                   p_vel_u ? p_vel_u->vector.data() : nullptr,
+                   p_vel_v ? p_vel_v->vector.data() : nullptr),
+        "hydraulic_vpipes");
+}
+
+} // namespace gpu
+} // namespace hmap

@@ -63,6 +63,24 @@ struct alignas(4 * sizeof(typename S::T)) Flux4
   typename S::T l, r, t, b;
 };
 
+// Element `idx` of a plane addressed as a uniform base plus an unsigned
+// 32-bit byte offset, so the compiler can use the scalar-base + 32-bit
+// vector-offset form of global loads and stores instead of per-lane 64-bit
+// address arithmetic.  Planes are < 4 GiB (grids are limited to 2^31 cells;
+// the largest element is a 16-byte flux quadruple for grids <= 2^28 cells).
+template <class T>
+__device__ __forceinline__ const T &at_off(const T *base, uint32_t idx)
+{
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      idx * (uint32_t)sizeof(T));
+}
+
+template <class T>
+__device__ __forceinline__ T &at_off(T *base, uint32_t idx)
+{
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T));
+}
+
 // ----------------------------------------------------- sin/cos and hashing
 
 // Project-written sin/cos: Cody-Waite reduction by pi/2 with a three-part
@@ -399,12 +417,11 @@ __device__ __forceinline__ TexelFetch vp_transport_issue(const typename S::T *sp
   t.w10 = a * (1.f - b);
   t.w01 = (1.f - a) * b;
   t.w11 = a * b;
-  const typename S::T *r0 = sp + j0 * k.nx;
-  const typename S::T *r1 = sp + j1 * k.nx;
-  t.s00 = S::ld(r0[i0]);
-  t.s10 = S::ld(r0[i1]);
-  t.s01 = S::ld(r1[i0]);
-  t.s11 = S::ld(r1[i1]);
+  const uint32_t r0 = (uint32_t)(j0 * k.nx), r1 = (uint32_t)(j1 * k.nx);
+  t.s00 = S::ld(at_off(sp, r0 + i0));
+  t.s10 = S::ld(at_off(sp, r0 + i1));
+  t.s01 = S::ld(at_off(sp, r1 + i0));
+  t.s11 = S::ld(at_off(sp, r1 + i1));
   return t;
 }
 

@@ -282,16 +282,16 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
                                             const typename S::T *__restrict__ D,
                                             const Flux4<S> *__restrict__ F,
                                             const float *__restrict__ rain,
-                                            int   row_off,
-                                            float topup_add,
-                                            float wta,
+                                            uint32_t row_off,
+                                            float    topup_add,
+                                            float    wta,
                                             const VpConst &k,
                                             const VpDyn   &y)
 {
   RowA     a;
-  Flux4<S> f = F[row_off];
-  a.z = z[row_off];
-  a.dr = S::ld(D[row_off]);
+  Flux4<S> f = at_off(F, row_off);
+  a.z = at_off(z, row_off);
+  a.dr = S::ld(at_off(D, row_off));
   a.fl = S::ld(f.l);
   a.fr = S::ld(f.r);
   a.ft = S::ld(f.t);
@@ -309,7 +309,7 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
     a.d1 = d;
   }
   else
-    a.d1 = vp_d1(a.dr, rain[row_off], y.has_prev, y.topup, wta, y.rf, k);
+    a.d1 = vp_d1(a.dr, at_off(rain, row_off), y.has_prev, y.topup, wta, y.rf, k);
   return a;
 }
 
@@ -362,7 +362,8 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
   const int j1 = min(j0 + chunk_rows, ny);
 
   auto load = [&](int r) {
-    return stream_load<S, URAIN>(z, D, F, rain, clampi(r, ny) * nx + gi, topup_add, wta, k, y);
+    return stream_load<S, URAIN>(z, D, F, rain, (uint32_t)(clampi(r, ny) * nx + gi), topup_add,
+                                 wta, k, y);
   };
   // B(r): transport of row r from A(r-1), A(r), A(r+1), in two halves
   auto transport_issue = [&](const RowA &am, const RowA &a, const RowA &ap, int r) {
@@ -372,7 +373,7 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
     {
       t.w00 = 1.f;
       t.w10 = t.w01 = t.w11 = 0.f;
-      t.s00 = S::ld(Sp[gj * nx + gi]);
+      t.s00 = S::ld(at_off(Sp, (uint32_t)(gj * nx + gi)));
       t.s10 = t.s01 = t.s11 = 0.f;
       return t;
     }
@@ -412,6 +413,7 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
   Fix128 acc{0, 0};
   RowA   apf;
   if (PF) apf = load(j0 + 2);
+#pragma unroll 2
   for (int r = j0; r < j1; r++)
   {
     RowA ap2;
@@ -442,16 +444,16 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
     vp_erosion<KEXP1>(a0.z, s_0, zr, zl, ap1.z, am1.z, sr, sl, s_hi, s_lo, d2, u, v, zn, sn, k);
     if (writer)
     {
-      const int c = r * nx + gx;
-      zo[c] = zn;
-      So[c] = S::st(sn);
-      Do[c] = S::st(d2);
+      const uint32_t c = (uint32_t)(r * nx + gx);
+      at_off(zo, c) = zn;
+      at_off(So, c) = S::st(sn);
+      at_off(Do, c) = S::st(d2);
       Flux4<S> o;
       o.l = S::st(c_0[0]);
       o.r = S::st(c_0[1]);
       o.t = S::st(c_0[2]);
       o.b = S::st(c_0[3]);
-      Fo[c] = o;
+      at_off(Fo, c) = o;
       if (k.maintain) fix_add(acc, fix_term(d2 * k.ce));
     }
 
@@ -821,6 +823,10 @@ static int session_create_common(hmg_ctx                 *ctx,
   s->p = prm;
   s->k = make_vp_const(prm, nx, ny);
   s->esize = storage == HMG_STORAGE_F32 ? 4 : 2;
+  // measured best launch shapes at 4096^2 (fused kernel): 64 rows per
+  // wavefront for FP32 state, 48 for 16-bit state; loads one row ahead
+  s->chunk_rows = storage == HMG_STORAGE_F32 ? 64 : 48;
+  s->prefetch = 1;
   const size_t n = (size_t)nx * ny;
   int          rc = HMG_OK;
   for (int q = 0; q < 2 && rc == HMG_OK; q++)

@@ -1,0 +1,150 @@
+// hmap_gpu.hpp -- C++ drop-in for HighMap's grid-erosion GPU functions.
+//
+// Declares, with the reference's names, signatures and default arguments:
+//   hmap::gpu::init_opencl        HighMap/include/highmap/opencl/gpu_opencl.hpp:30
+//   hmap::gpu::noise_fbm          HighMap/include/highmap/primitives/coherent_noise.hpp:821-834
+//   hmap::gpu::thermal (3 overloads)  HighMap/include/highmap/erosion.hpp:1784-1801
+//   hmap::gpu::hydraulic_vpipes   HighMap/include/highmap/erosion.hpp:1206-1223
+// plus the minimal hmap::Array (HighMap/include/highmap/array.hpp:31-44,
+// 236-239: float storage, shape {nx, ny}, index j*nx + i) and hmap::NoiseType
+// (HighMap/include/highmap/functions.hpp:66-81) those signatures use.
+//
+// The functions run on libhmg (include/hmg.h) on the first HIP device.  The
+// reference returns void and reports nothing; here a failure throws
+// std::runtime_error carrying hmg_last_error(), so no result is silently
+// left unchanged.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <vector>
+
+#if __has_include(<glm/glm.hpp>)
+#include <glm/glm.hpp>
+#else
+// The few glm types the signatures use, when glm itself is not installed.
+namespace glm
+{
+struct ivec2
+{
+  int x = 0, y = 0;
+  ivec2() = default;
+  ivec2(int a, int b) : x(a), y(b) {}
+};
+struct vec2
+{
+  float x = 0.f, y = 0.f;
+  vec2() = default;
+  vec2(float a, float b) : x(a), y(b) {}
+};
+struct vec4
+{
+  float x = 0.f, y = 0.f, z = 0.f, w = 0.f;
+  vec4() = default;
+  vec4(float a, float b, float c, float d) : x(a), y(b), z(c), w(d) {}
+};
+} // namespace glm
+#endif
+
+namespace hmap
+{
+
+enum NoiseType : int
+{
+  PARBERRY,
+  PERLIN,
+  PERLIN_BILLOW,
+  PERLIN_HALF,
+  SIMPLEX2,
+  SIMPLEX2S,
+  VALUE,
+  VALUE_CUBIC,
+  VALUE_DELAUNAY,
+  VALUE_LINEAR,
+  WORLEY,
+  WORLEY_DOUBLE,
+  WORLEY_VALUE,
+};
+
+// 2-D float grid with hmap::Array's layout: shape {ni, nj}, index j*ni + i.
+class Array
+{
+public:
+  glm::ivec2         shape = glm::ivec2(0, 0);
+  std::vector<float> vector;
+
+  Array() = default;
+  explicit Array(glm::ivec2 s) : shape(s), vector((size_t)s.x * (size_t)s.y, 0.f) {}
+  Array(glm::ivec2 s, float value) : shape(s), vector((size_t)s.x * (size_t)s.y, value) {}
+
+  float &operator()(int i, int j) { return vector[(size_t)j * shape.x + i]; }
+  const float &operator()(int i, int j) const { return vector[(size_t)j * shape.x + i]; }
+  size_t       size() const { return vector.size(); }
+  float        sum() const;
+  float        min() const;
+  float        max() const;
+};
+
+// hmap::remap (HighMap/src/range/range.cpp:342-361).
+void remap(Array &array, float vmin = 0.f, float vmax = 1.f);
+
+namespace gpu
+{
+
+bool init_opencl();
+
+Array noise_fbm(NoiseType     noise_type,
+                glm::ivec2    shape,
+                glm::vec2     kw,
+                std::uint32_t seed,
+                int           octaves = 8,
+                float         weight = 0.7f,
+                float         persistence = 0.5f,
+                float         lacunarity = 2.f,
+                const Array  *p_ctrl_param = nullptr,
+                const Array  *p_noise_x = nullptr,
+                const Array  *p_noise_y = nullptr,
+                const Array  *p_stretching = nullptr,
+                glm::vec4     bbox = {0.f, 1.f, 0.f, 1.f},
+                glm::ivec2    period = {0, 0});
+
+void thermal(Array       &z,
+             const Array &talus,
+             int          iterations = 10,
+             Array       *p_bedrock = nullptr,
+             Array       *p_deposition_map = nullptr);
+
+void thermal(Array       &z,
+             const Array *p_mask,
+             const Array &talus,
+             int          iterations = 10,
+             Array       *p_bedrock = nullptr,
+             Array       *p_deposition_map = nullptr);
+
+void thermal(Array &z,
+             float  talus,
+             int    iterations = 10,
+             Array *p_bedrock = nullptr,
+             Array *p_deposition_map = nullptr);
+
+void hydraulic_vpipes(Array &z,
+                      float  water_height = 1e-2f,
+                      bool   maintain_water_volume = true,
+                      float  evap_rate = 0.1f,
+                      int    iterations = 50,
+                      float  dt = 0.5f,
+                      float  k_capacity = 0.5f,
+                      float  k_erode = 0.001f,
+                      float  k_depose = 0.01f,
+                      float  k_discharge_exp = 1.f,
+                      float  downcutting_max_depth_ratio = 10.f,
+                      bool   flux_diffusion = true,
+                      float  flux_diffusion_strength = 0.01f,
+                      Array *p_rain_map = nullptr,
+                      Array *p_water_depth = nullptr,
+                      Array *p_sediment = nullptr,
+                      Array *p_vel_u = nullptr,
+                      Array *p_vel_v = nullptr);
+
+} // namespace gpu
+} // namespace hmap

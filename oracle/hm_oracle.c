@@ -394,6 +394,60 @@ uint64_t orc_xorshift_period(uint32_t start)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Storage rounding (round to nearest even), written from the IEEE-754       */
+/* binary16 and bfloat16 formats.  Parity pinned: exhaustive comparison with */
+/* numpy's float16 conversion and a bit-level bfloat16 restatement over all  */
+/* 2^32 float32 patterns in tests/test_oracle_vpipes.py (strided sample).    */
+/* ------------------------------------------------------------------------ */
+
+float orc_round_bf16(float x)
+{
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  if ((b & 0x7f800000u) == 0x7f800000u) return x; /* inf, nan */
+  b += 0x7fffu + ((b >> 16) & 1u);
+  b &= 0xffff0000u;
+  float r;
+  memcpy(&r, &b, 4);
+  return r;
+}
+
+float orc_round_f16(float x)
+{
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  uint32_t sign = b & 0x80000000u, a = b & 0x7fffffffu;
+  float    r;
+  if (a >= 0x7f800000u) return x;             /* inf, nan */
+  if (a >= 0x477ff000u)                       /* >= 65520: rounds to inf */
+    return sign ? -INFINITY : INFINITY;
+  if (a < 0x38800000u)                        /* below 2^-14: subnormal half */
+  {
+    float ax;
+    memcpy(&ax, &a, 4);
+    r = rintf(ax * 0x1p24f) * 0x1p-24f;       /* quantum 2^-24, exact scaling */
+  }
+  else
+  {
+    a += 0x0fffu + ((a >> 13) & 1u);          /* keep 10 stored mantissa bits */
+    a &= ~0x1fffu;
+    memcpy(&r, &a, 4);
+  }
+  return sign ? -r : r;
+}
+
+static float orc_store_f32(float x, int storage)
+{
+  return storage == 1 ? orc_round_bf16(x) : storage == 2 ? orc_round_f16(x) : x;
+}
+
+static double orc_store_f64(double x, int storage)
+{
+  (void)storage; /* the FP64 oracle keeps every plane in FP64 */
+  return x;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Thermal and vpipes, instantiated for float and double.                    */
 /* ------------------------------------------------------------------------ */
 
@@ -452,6 +506,7 @@ static double orc_sum_f32fix(const float *a, int nx, int ny)
 #define R_HYPOT orc_hypotf
 #define R_POW orc_powf
 #define R_VOLUME orc_sum_f32fix
+#define R_STORE orc_store_f32
 #define R_IS_F32 1
 #include "hm_oracle_erosion.inc"
 #undef REAL
@@ -466,6 +521,7 @@ static double orc_sum_f32fix(const float *a, int nx, int ny)
 #undef R_HYPOT
 #undef R_POW
 #undef R_VOLUME
+#undef R_STORE
 #undef R_IS_F32
 
 #define REAL double
@@ -480,5 +536,6 @@ static double orc_sum_f32fix(const float *a, int nx, int ny)
 #define R_HYPOT orc_hypotd
 #define R_POW orc_powd
 #define R_VOLUME orc_sum_f64
+#define R_STORE orc_store_f64
 #define R_IS_F32 0
 #include "hm_oracle_erosion.inc"

@@ -57,7 +57,16 @@ typedef struct orc_vp_params
   float rain_per_iter;
   int   maintain_water_volume;
   int   flux_diffusion;
+  /* Storage of the water, sediment and flux state (FP32 instance only):
+   * 0 FP32, 1 BF16, 2 FP16.  Each value of those planes is rounded to the
+   * storage type as soon as it is produced (initial water, flow-pass
+   * fluxes, water-pass depth, erosion-pass sediment) and every consumer uses
+   * the rounded value; arithmetic stays FP32. */
+  int   storage;
 } orc_vp_params;
+
+float orc_round_bf16(float x);
+float orc_round_f16(float x);
 
 /* --- integer seed path and project-written transcendental functions --- */
 uint32_t orc_wang_hash(uint32_t seed);

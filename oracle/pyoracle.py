@@ -35,6 +35,7 @@ class VpParams(C.Structure):
         ("rain_per_iter", C.c_float),
         ("maintain_water_volume", C.c_int),
         ("flux_diffusion", C.c_int),
+        ("storage", C.c_int),
     ]
 
 
@@ -53,7 +54,10 @@ VP_DEFAULTS = dict(
     rain_per_iter=0.0,
     maintain_water_volume=1,
     flux_diffusion=1,
+    storage=0,
 )
+
+_INT_FIELDS = ("maintain_water_volume", "flux_diffusion", "storage")
 
 
 def vp_params(**kw):
@@ -62,8 +66,7 @@ def vp_params(**kw):
         if k not in d:
             raise KeyError(k)
         d[k] = v
-    return VpParams(**{k: (int(v) if k in ("maintain_water_volume", "flux_diffusion") else float(v))
-                       for k, v in d.items()})
+    return VpParams(**{k: (int(v) if k in _INT_FIELDS else float(v)) for k, v in d.items()})
 
 
 def build():
@@ -107,6 +110,9 @@ def lib():
                                    C.POINTER(C.c_int64)]
     L.orc_xorshift_period.argtypes = [u32]
     L.orc_xorshift_period.restype = C.c_uint64
+    for name in ("orc_round_bf16", "orc_round_f16"):
+        getattr(L, name).argtypes = [C.c_float]
+        getattr(L, name).restype = C.c_float
     pp = C.POINTER(VpParams)
     dp = C.POINTER(C.c_double)
     L.orc_vpipes_init_f32.argtypes = [F32P] * 8 + [C.c_int, C.c_int, pp, C.c_void_p, dp, dp]

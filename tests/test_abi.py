@@ -54,6 +54,31 @@ def test_library_loads_and_answers_without_gpu():
     assert p.gravity == 1.0 and p.rain_per_iter == 0.0 and p.mode == 0
 
 
+def test_cpp_dropin_exports_reference_signatures():
+    """libhmap_gpu.so defines hmap::gpu::* with the reference's parameter
+    lists (HighMap/include/highmap/erosion.hpp:1206-1223,1784-1801,
+    coherent_noise.hpp:821-834)."""
+    lib = os.path.join(ROOT, "highmap_amd", "libhmap_gpu.so")
+    assert os.path.exists(lib), "libhmap_gpu.so not built"
+    out = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True,
+                         check=True).stdout
+    want = [
+        "hmap::gpu::init_opencl()",
+        "hmap::gpu::noise_fbm(hmap::NoiseType, glm::ivec2, glm::vec2, unsigned int, int, float, "
+        "float, float, hmap::Array const*, hmap::Array const*, hmap::Array const*, "
+        "hmap::Array const*, glm::vec4, glm::ivec2)",
+        "hmap::gpu::thermal(hmap::Array&, hmap::Array const&, int, hmap::Array*, hmap::Array*)",
+        "hmap::gpu::thermal(hmap::Array&, hmap::Array const*, hmap::Array const&, int, "
+        "hmap::Array*, hmap::Array*)",
+        "hmap::gpu::thermal(hmap::Array&, float, int, hmap::Array*, hmap::Array*)",
+        "hmap::gpu::hydraulic_vpipes(hmap::Array&, float, bool, float, int, float, float, float, "
+        "float, float, float, bool, float, hmap::Array*, hmap::Array*, hmap::Array*, "
+        "hmap::Array*, hmap::Array*)",
+    ]
+    for w in want:
+        assert w in out, w
+
+
 def test_session_bytes_accounting():
     import highmap_amd
 

@@ -203,6 +203,36 @@ def test_session_vpipes_thermal_interleaved(hm, orc, mode):
     _assert_same(g, (o.z, o.d, o.s, o.u, o.v))
 
 
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+@pytest.mark.parametrize("storage", [1, 2])
+def test_session_16bit_storage_bitexact(hm, orc, mode, storage):
+    """BF16 (1) / FP16 (2) storage of water, sediment and fluxes with FP32
+    arithmetic: identical to the oracle with the same storage rounding."""
+    nx, ny = 150, 110
+    z0 = _terrain(orc, nx, ny, 12)
+    s = hm.Session(z0, mode=mode, storage=storage, **VP_WORKLOAD)
+    o = orc.VpState(z0, orc.vp_params(storage=storage, **VP_WORKLOAD))
+    for _ in range(3):
+        s.vpipes(7).thermal(1.0 / nx, 1)
+        o.run(7).thermal_step(1.0 / nx, 1)
+    g = s.export()
+    s.close()
+    _assert_same(g, (o.z, o.d, o.s, o.u, o.v))
+
+
+def test_fused_launch_shapes_do_not_change_results(hm, orc):
+    z0 = _terrain(orc, 300, 170, 13)
+    outs = []
+    for ch, pf in ((1, 0), (7, 1), (32, 0), (64, 1), (200, 1)):
+        s = hm.Session(z0, **VP_WORKLOAD)
+        s.set_tuning(ch, pf)
+        s.vpipes(9)
+        outs.append(s.export())
+        s.close()
+    for o in outs[1:]:
+        _assert_same(o, outs[0])
+
+
 def test_plain_and_fused_agree_4096(hm, orc):
     """At full size: 6 iterations plain vs fused are identical, and the water
     volume stays at W0 (maintain_water_volume) to FP32 rounding."""

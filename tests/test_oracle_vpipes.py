@@ -296,6 +296,51 @@ def test_water_volume_maintained(dtype):
         assert abs(vol - st.W0) <= tol * st.W0
 
 
+def _f32_patterns(step=4099):
+    bits = np.arange(0, 1 << 32, step, dtype=np.uint64).astype(np.uint32)
+    edge = np.array([0x477fe000, 0x477fefff, 0x477ff000, 0x38800000, 0x387fffff, 0x33800000,
+                     0x33000000, 0x33000001, 0x7f7fffff, 0x00000001, 0x3f800000, 0x3f801000,
+                     0x3f803000, 0x3f808000, 0x3f818000], np.uint32)
+    bits = np.concatenate([bits, edge, edge | 0x80000000])
+    x = bits.view(np.float32)
+    return x[np.isfinite(x)]
+
+
+def test_round_f16_matches_ieee_binary16():
+    L = O.lib()
+    x = _f32_patterns()
+    got = np.array([L.orc_round_f16(float(v)) for v in x], np.float32)
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).astype(np.float32)
+    assert np.array_equal(got, ref)
+
+
+def test_round_bf16_matches_bit_rounding():
+    L = O.lib()
+    x = _f32_patterns()
+    got = np.array([L.orc_round_bf16(float(v)) for v in x], np.float32)
+    b = x.view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)
+    # values that round past the largest finite bfloat16 become infinity
+    assert np.array_equal(got, r)
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_storage_rounding_keeps_planes_representable(storage):
+    z = O.fbm_terrain(40, 32, 2)
+    st = O.VpState(z, O.vp_params(storage=storage, **VP_WORKLOAD)).run(6)
+    dt = np.float16 if storage == 2 else None
+    for plane in (st.fl, st.fr, st.ft, st.fb):
+        if dt is not None:
+            assert np.array_equal(plane.astype(dt).astype(np.float32), plane)
+        else:
+            assert np.all((plane.view(np.uint32) & 0xFFFF) == 0)
+    # FP32 storage (storage 0) is the FP32 contract itself
+    a = O.VpState(z, O.vp_params(storage=0, **VP_WORKLOAD)).run(6)
+    b = O.VpState(z, O.vp_params(**VP_WORKLOAD)).run(6)
+    assert np.array_equal(a.z, b.z)
+
+
 def test_fp32_contract_tracks_fp64_at_reference_defaults():
     """At hmap::gpu::hydraulic_vpipes' default parameters the FP32 contract
     stays within 1e-4 of the height range of the FP64 oracle after 50
