@@ -46,7 +46,16 @@ struct StF16
 {
   using T = _Float16;
   static __device__ __forceinline__ float ld(T x) { return (float)x; }
-  static __device__ __forceinline__ T     st(float x) { return (_Float16)x; }
+  static __device__ __forceinline__ T     st(float x)
+  {
+    // The empty asm makes x an FP32 value in a register before the
+    // conversion.  Without it the compiler may merge the FP32 multiply or
+    // add that produced x with the conversion into one mixed-precision
+    // instruction (v_fma_mixlo_f16), which rounds once to FP16 instead of
+    // first to FP32 and then to FP16, and so changes results.
+    asm volatile("" : "+v"(x));
+    return (_Float16)x;
+  }
 };
 
 // Rounds a value to what the storage type would hold.

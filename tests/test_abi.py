@@ -79,6 +79,29 @@ def test_cpp_dropin_exports_reference_signatures():
         assert w in out, w
 
 
+@pytest.mark.slow
+def test_device_code_has_no_single_rounding_mixed_precision_ops():
+    """The kernels must round to FP32 before any FP16 conversion; the
+    compiler's mixed-precision fma (v_fma_mix*/v_mad_mix*) rounds once and
+    would change results.  Checks the gfx950 assembly of every source."""
+    import shutil
+
+    from highmap_amd import build as hb
+
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    for src in ("noise.hip", "thermal.hip", "vpipes.hip", "batch.hip"):
+        path = os.path.join(hb.CSRC, src)
+        if not os.path.exists(path):
+            continue
+        flags = [f for f in hb.HIPCC_FLAGS if f != "-fPIC"]
+        asm = subprocess.run([hipcc] + flags + ["-x", "hip", "-S", "--cuda-device-only", path,
+                                                "-o", "-"], capture_output=True, text=True,
+                             check=True).stdout
+        assert not re.search(r"\bv_(fma|mad)_mix", asm), src
+
+
 def test_session_bytes_accounting():
     import highmap_amd
 
