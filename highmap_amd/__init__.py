@@ -134,6 +134,13 @@ def lib():
             "hmg_session_sync": ([vp], i32),
             "hmg_session_destroy": ([vp], i32),
             "hmg_session_bytes": ([i32, i32, i32], sz),
+            "hmg_batch_create": ([vp, i32, pp, C.POINTER(vp)], i32),
+            "hmg_batch_add": ([vp, vp, i32, i32, vp, i32, C.POINTER(i32)], i32),
+            "hmg_batch_set_tuning": ([vp, i32, i32], i32),
+            "hmg_batch_autotune": ([vp, i32, C.POINTER(i32), C.POINTER(i32)], i32),
+            "hmg_batch_run": ([vp, i32, C.POINTER(f32)], i32),
+            "hmg_batch_export": ([vp, i32, vp, vp, vp, vp, vp], i32),
+            "hmg_batch_destroy": ([vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name, None)
@@ -351,6 +358,67 @@ class Session:
     def close(self):
         if getattr(self, "handle", None):
             lib().hmg_session_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent maps advanced together by batched launches
+    (hmg_batch_*); each map's result equals running it alone."""
+
+    def __init__(self, storage=Storage.F32, params=None, ctx=None, **kw):
+        self.ctx = _ctx(ctx)
+        self.params = params if params is not None else VpipesParams.make(**kw)
+        h = C.c_void_p()
+        _check(lib().hmg_batch_create(self.ctx.handle, int(storage), C.byref(self.params),
+                                      C.byref(h)))
+        self.handle = h
+        self.shapes = []
+
+    def add(self, z, iterations, rain_map=None):
+        z = _f32(z)
+        ny, nx = z.shape
+        r = _f32(rain_map, (ny, nx), "rain_map")
+        mid = C.c_int()
+        _check(lib().hmg_batch_add(self.handle, _ptr(z), nx, ny, _ptr(r), int(iterations),
+                                   C.byref(mid)))
+        self.shapes.append((ny, nx))
+        return mid.value
+
+    def set_tuning(self, chunk_rows, prefetch=1):
+        _check(lib().hmg_batch_set_tuning(self.handle, int(chunk_rows), int(prefetch)))
+
+    def autotune(self, probe=2):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().hmg_batch_autotune(self.handle, int(probe), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def run(self, use_graph=True):
+        """Runs every map to its iteration count; returns device ms."""
+        ms = C.c_float()
+        _check(lib().hmg_batch_run(self.handle, int(bool(use_graph)), C.byref(ms)))
+        return ms.value
+
+    def export(self, map_id):
+        shp = self.shapes[map_id]
+        out = [np.empty(shp, np.float32) for _ in range(5)]
+        _check(lib().hmg_batch_export(self.handle, int(map_id), *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().hmg_batch_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

@@ -34,6 +34,8 @@
 //                  the plain form: one launch per pass, every intermediate
 //                  through HBM, as the reference structures its loop.
 //   k_vp_export    forms d, s, u, v as hydraulic_vpipes returns them.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "hmg_internal.hpp"
@@ -309,39 +311,44 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
     a.d1 = d;
   }
   else
-    a.d1 = vp_d1(a.dr, at_off(rain, row_off), y.has_prev, y.topup, wta, y.rf, k);
+    a.d1 = vp_d1(a.dr, rain ? at_off(rain, row_off) : 1.f, y.has_prev, y.topup, wta, y.rf, k);
   return a;
 }
 
+// XCD-aware work order: workgroup b runs on XCD b % 8; give each XCD a
+// contiguous range of work items so halo reuse stays in its L2.
+__device__ __forceinline__ int xcd_wave_item()
+{
+  const int G = gridDim.x;
+  const int b = (G & 7) ? blockIdx.x : (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  return b * 4 + (threadIdx.x >> 6);
+}
+
+// One wavefront's work item of one iteration of one map: a strip of 60
+// output columns over `chunk_rows` rows.  `item` numbers the map's items;
+// the wavefront of item 0 clears the slot set of the next iteration.
 template <class S, bool URAIN, bool PF, bool KEXP1>
-__global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
-                                                   const typename S::T *__restrict__ D,
-                                                   const Flux4<S> *__restrict__ F,
-                                                   const typename S::T *__restrict__ Sp,
-                                                   const float *__restrict__ rain,
-                                                   float *__restrict__ zo,
-                                                   typename S::T *__restrict__ Do,
-                                                   Flux4<S> *__restrict__ Fo,
-                                                   typename S::T *__restrict__ So,
-                                                   const Fix128 *__restrict__ prev_set,
-                                                   Fix128 *set,
-                                                   Fix128 *clear_set,
-                                                   int     strips,
-                                                   int     chunk_rows,
-                                                   int     n_items,
-                                                   VpConst k,
-                                                   VpDyn   y)
+__device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
+                                               const typename S::T *__restrict__ D,
+                                               const Flux4<S> *__restrict__ F,
+                                               const typename S::T *__restrict__ Sp,
+                                               const float *__restrict__ rain,
+                                               float *__restrict__ zo,
+                                               typename S::T *__restrict__ Do,
+                                               Flux4<S> *__restrict__ Fo,
+                                               typename S::T *__restrict__ So,
+                                               const Fix128 *__restrict__ prev_set,
+                                               Fix128 *set,
+                                               Fix128 *clear_set,
+                                               int     item,
+                                               int     strips,
+                                               int     chunk_rows,
+                                               const VpConst &k,
+                                               const VpDyn   &y)
 {
   const int nx = k.nx, ny = k.ny;
   const int lane = threadIdx.x & 63;
-  if (k.maintain) clear_slots(clear_set);
-
-  // XCD-aware work order: workgroup b runs on XCD b % 8; give each XCD a
-  // contiguous range of (chunk, strip) items so halo reuse stays in its L2.
-  const int G = gridDim.x;
-  const int b = (G & 7) ? blockIdx.x : (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
-  const int item = b * 4 + (threadIdx.x >> 6);
-  if (item >= n_items) return;
+  if (k.maintain && item == 0 && lane < NSLOT) clear_set[lane] = Fix128{0, 0};
   const int strip = item % strips, chunk = item / strips;
 
   float wta = 0.f, topup_add = 0.f;
@@ -488,6 +495,93 @@ __global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
       atomicAdd((unsigned long long *)&slot->hi, (unsigned long long)acc.hi + carry);
     }
   }
+}
+
+// One iteration of one session.
+template <class S, bool URAIN, bool PF, bool KEXP1>
+__global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
+                                                   const typename S::T *__restrict__ D,
+                                                   const Flux4<S> *__restrict__ F,
+                                                   const typename S::T *__restrict__ Sp,
+                                                   const float *__restrict__ rain,
+                                                   float *__restrict__ zo,
+                                                   typename S::T *__restrict__ Do,
+                                                   Flux4<S> *__restrict__ Fo,
+                                                   typename S::T *__restrict__ So,
+                                                   const Fix128 *__restrict__ prev_set,
+                                                   Fix128 *set,
+                                                   Fix128 *clear_set,
+                                                   int     strips,
+                                                   int     chunk_rows,
+                                                   int     n_items,
+                                                   VpConst k,
+                                                   VpDyn   y)
+{
+  const int item = xcd_wave_item();
+  if (item >= n_items) return;
+  vp_stream_item<S, URAIN, PF, KEXP1>(z, D, F, Sp, rain, zo, Do, Fo, So, prev_set, set,
+                                      clear_set, item, strips, chunk_rows, k, y);
+}
+
+// ------------------------------------------------------------ batched form
+//
+// One launch advances every map of a batch by one iteration.  The maps are
+// described by a table in device memory; each map keeps its own planes (both
+// ping-pong halves), slot sets and constants, so its result does not depend
+// on which other maps share the launch.  Map m runs its iteration
+// it0 + (t - t_start) at launch t while that is below it0 + iters; the
+// parity of its planes and its slot sets follow from that iteration number.
+
+struct VpMapDesc
+{
+  void        *z[2], *D[2], *F[2], *Sp[2];
+  const float *rain;
+  Fix128      *slots; // 3 sets of NSLOT
+  VpConst      k;
+  double       W0;
+  float        rf;
+  int          topup;
+  int          strips, chunk_rows, n_items, item_base;
+  int          t_start, it0, iters;
+};
+
+template <class S, bool URAIN, bool PF, bool KEXP1>
+__global__ void __launch_bounds__(256) k_vp_batch(const VpMapDesc *__restrict__ maps,
+                                                  int n_maps,
+                                                  int n_items,
+                                                  int t)
+{
+  using T = typename S::T;
+  const int item = xcd_wave_item();
+  if (item >= n_items) return;
+  // the map holding this item: maps are ordered by item_base
+  int lo = 0, hi = n_maps - 1;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (maps[mid].item_base <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const VpMapDesc &m = maps[lo];
+  const int        rel = t - m.t_start;
+  if (rel < 0 || rel >= m.iters) return;
+  const int mi = m.it0 + rel;
+  const int a = mi & 1, b = a ^ 1;
+  VpDyn     y;
+  y.has_prev = mi > 0;
+  y.topup = m.topup;
+  y.W0 = m.W0;
+  y.rf = m.rf;
+  Fix128 *prev = m.slots + ((mi + 2) % 3) * NSLOT;
+  Fix128 *cur = m.slots + (mi % 3) * NSLOT;
+  Fix128 *nxt = m.slots + ((mi + 1) % 3) * NSLOT;
+  vp_stream_item<S, URAIN, PF, KEXP1>((const float *)m.z[a], (const T *)m.D[a],
+                                      (const Flux4<S> *)m.F[a], (const T *)m.Sp[a], m.rain,
+                                      (float *)m.z[b], (T *)m.D[b], (Flux4<S> *)m.F[b],
+                                      (T *)m.Sp[b], prev, cur, nxt, item - m.item_base, m.strips,
+                                      m.chunk_rows, m.k, y);
 }
 
 // ---------------------------------------------------------------- export
@@ -1078,6 +1172,340 @@ int hmg_vpipes(hmg_ctx                 *ctx,
                                             v_out_host);
   hmg_session_destroy(s);
   return rc;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------ batch
+
+namespace hmg
+{
+
+// Advances sessions[i] by iters[i] iterations with batched launches (one
+// launch per iteration for all maps still running).  All sessions share one
+// storage type.  With use_graph the whole launch sequence is captured in a
+// hipGraph and submitted as one graph.  Returns the elapsed device time of
+// the launches in *ms (if not null).
+template <class S, bool PF, bool KEXP1>
+static int batch_advance_t(hmg_ctx           *ctx,
+                           hmg_session *const *ss,
+                           const int          *iters,
+                           int                 n,
+                           int                 use_graph,
+                           int                 chunk_rows,
+                           float              *ms)
+{
+  hipStream_t      st = ctx->stream;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return iters[a] > iters[b]; });
+  std::vector<VpMapDesc> d(n);
+  std::vector<int>       prefix(n + 1, 0); // items of the first k maps
+  bool                   any_rain = false;
+  int                    T = 0;
+  for (int q = 0; q < n; q++)
+  {
+    hmg_session *s = ss[order[q]];
+    VpMapDesc   &m = d[q];
+    for (int h = 0; h < 2; h++)
+    {
+      m.z[h] = s->z[h].ptr;
+      m.D[h] = s->D[h].ptr;
+      m.F[h] = s->F[h].ptr;
+      m.Sp[h] = s->Sp[h].ptr;
+    }
+    m.rain = s->rain.as<float>();
+    any_rain = any_rain || m.rain;
+    m.slots = s->slots.as<Fix128>();
+    m.k = s->k;
+    m.W0 = s->W0;
+    m.rf = s->rf;
+    m.topup = s->p.maintain_water_volume ? 1 : 0;
+    m.strips = (s->nx + STRIP_OUT - 1) / STRIP_OUT;
+    m.chunk_rows = chunk_rows < s->ny ? chunk_rows : s->ny;
+    m.n_items = m.strips * ((s->ny + m.chunk_rows - 1) / m.chunk_rows);
+    m.item_base = prefix[q];
+    prefix[q + 1] = prefix[q] + m.n_items;
+    m.t_start = 0;
+    m.it0 = (int)s->iter;
+    m.iters = iters[order[q]];
+    if (m.iters > T) T = m.iters;
+  }
+  DevBuf table;
+  HMG_TRY(table.get(ctx->pool, sizeof(VpMapDesc) * (size_t)(n > 0 ? n : 1)));
+  if (n > 0)
+    HMG_HIP(hipMemcpyAsync(table.ptr, d.data(), sizeof(VpMapDesc) * n, hipMemcpyHostToDevice, st));
+  HMG_HIP(hipStreamSynchronize(st)); // d (pageable) may be released after this
+
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms)
+  {
+    HMG_HIP(hipEventCreate(&e0));
+    HMG_HIP(hipEventCreate(&e1));
+  }
+  hipGraph_t     graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) HMG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  else if (ms)
+    HMG_HIP(hipEventRecord(e0, st));
+  int active = n;
+  for (int t = 0; t < T; t++)
+  {
+    while (active > 0 && d[active - 1].iters <= t) active--;
+    const int items = prefix[active];
+    const int G = (items + 3) / 4;
+    if (G == 0) continue;
+    const VpMapDesc *tab = table.as<VpMapDesc>();
+    if (any_rain)
+      k_vp_batch<S, false, PF, KEXP1><<<G, 256, 0, st>>>(tab, active, items, t);
+    else
+      k_vp_batch<S, true, PF, KEXP1><<<G, 256, 0, st>>>(tab, active, items, t);
+  }
+  int rc = check_launch("vp_batch");
+  if (use_graph)
+  {
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc != HMG_OK || e != hipSuccess)
+    {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc != HMG_OK ? rc : set_error(HMG_ERR_DEVICE, "graph capture failed: %s",
+                                           hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess && ms) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, st);
+    if (e != hipSuccess)
+    {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      return set_error(HMG_ERR_DEVICE, "graph launch failed: %s", hipGetErrorString(e));
+    }
+  }
+  if (ms) HMG_HIP(hipEventRecord(e1, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (ms)
+  {
+    HMG_HIP(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  HMG_TRY(rc);
+  for (int i = 0; i < n; i++)
+  {
+    ss[i]->iter += iters[i];
+    ss[i]->cur = (int)(ss[i]->iter & 1);
+  }
+  return HMG_OK;
+}
+
+int batch_advance(hmg_ctx           *ctx,
+                  hmg_session *const *ss,
+                  const int          *iters,
+                  int                 n,
+                  int                 use_graph,
+                  int                 chunk_rows,
+                  int                 prefetch,
+                  float              *ms)
+{
+  if (n <= 0) return HMG_OK;
+  const int  storage = ss[0]->storage;
+  const bool ke1 = ss[0]->k.kexp == 1.f;
+  for (int i = 0; i < n; i++)
+  {
+    if (ss[i]->storage != storage)
+      return set_error(HMG_ERR_INVALID, "batched maps must share one storage type");
+    if ((ss[i]->k.kexp == 1.f) != ke1)
+      return set_error(HMG_ERR_INVALID, "batched maps must share k_discharge_exp == 1 or != 1");
+    if (iters[i] < 0) return set_error(HMG_ERR_INVALID, "negative iteration count");
+    if ((long)ss[i]->iter + iters[i] > (1L << 30))
+      return set_error(HMG_ERR_INVALID, "iteration count too large");
+  }
+  if (chunk_rows < 1) return set_error(HMG_ERR_INVALID, "chunk_rows must be >= 1");
+#define HMG_BATCH(ST)                                                          \
+  (prefetch ? (ke1 ? batch_advance_t<ST, true, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms) \
+                   : batch_advance_t<ST, true, false>(ctx, ss, iters, n, use_graph, chunk_rows, ms)) \
+            : (ke1 ? batch_advance_t<ST, false, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms) \
+                   : batch_advance_t<ST, false, false>(ctx, ss, iters, n, use_graph, chunk_rows, ms)))
+  switch (storage)
+  {
+  case HMG_STORAGE_BF16: return HMG_BATCH(StBF16);
+  case HMG_STORAGE_F16: return HMG_BATCH(StF16);
+  default: return HMG_BATCH(StF32);
+  }
+#undef HMG_BATCH
+}
+
+} // namespace hmg
+
+// Autotuner cache: best (chunk_rows, prefetch) per (storage, size class),
+// size class = floor(log2(total cells of the batch)).
+namespace
+{
+std::mutex                                     g_tune_mu;
+std::map<std::pair<int, int>, std::pair<int, int>> g_tune;
+} // namespace
+
+struct hmg_batch
+{
+  hmg_ctx                  *ctx = nullptr;
+  int                       storage = HMG_STORAGE_F32;
+  hmg_vpipes_params         p{};
+  std::vector<hmg_session *> maps;
+  std::vector<int>          iters;
+  int                       chunk_rows = 48;
+  int                       prefetch = 1;
+  float                     last_ms = 0.f;
+};
+
+extern "C" {
+
+int hmg_batch_create(hmg_ctx *ctx, int storage, const hmg_vpipes_params *p, hmg_batch **out)
+{
+  clear_error();
+  if (!ctx || !out) return set_error(HMG_ERR_INVALID, "null argument");
+  if (storage != HMG_STORAGE_F32 && storage != HMG_STORAGE_BF16 && storage != HMG_STORAGE_F16)
+    return set_error(HMG_ERR_INVALID, "unknown storage %d", storage);
+  hmg_batch *b = new hmg_batch();
+  b->ctx = ctx;
+  b->storage = storage;
+  if (p)
+    b->p = *p;
+  else
+    hmg_vpipes_defaults(&b->p);
+  b->p.mode = HMG_VP_FUSED;
+  int rc = validate_vp_params(b->p);
+  if (rc != HMG_OK)
+  {
+    delete b;
+    return rc;
+  }
+  b->chunk_rows = storage == HMG_STORAGE_F32 ? 64 : 48;
+  *out = b;
+  return HMG_OK;
+}
+
+int hmg_batch_add(hmg_batch   *b,
+                  const float *z_host,
+                  int          nx,
+                  int          ny,
+                  const float *rain_host,
+                  int          iterations,
+                  int         *map_id)
+{
+  clear_error();
+  if (!b) return set_error(HMG_ERR_INVALID, "null batch");
+  if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
+  hmg_session *s = nullptr;
+  HMG_TRY(hmg_session_create(b->ctx, z_host, nx, ny, &b->p, rain_host, b->storage, &s));
+  b->maps.push_back(s);
+  b->iters.push_back(iterations);
+  if (map_id) *map_id = (int)b->maps.size() - 1;
+  return HMG_OK;
+}
+
+int hmg_batch_set_tuning(hmg_batch *b, int chunk_rows, int prefetch)
+{
+  clear_error();
+  if (!b) return set_error(HMG_ERR_INVALID, "null batch");
+  if (chunk_rows < 1 || (prefetch != 0 && prefetch != 1))
+    return set_error(HMG_ERR_INVALID, "bad tuning values");
+  b->chunk_rows = chunk_rows;
+  b->prefetch = prefetch;
+  return HMG_OK;
+}
+
+// Times each candidate launch shape on `probe` iterations of the live maps
+// (results do not depend on the shape, so the probe iterations are part of
+// the run) and keeps the fastest; the choice is cached per storage type and
+// size class for later batches.
+int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefetch_out)
+{
+  clear_error();
+  if (!b) return set_error(HMG_ERR_INVALID, "null batch");
+  if (b->maps.empty()) return HMG_OK;
+  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  HMG_TRY(ctx_enter(b->ctx));
+  double cells = 0;
+  for (hmg_session *s : b->maps) cells += (double)s->nx * s->ny;
+  const int key_size = (int)std::log2(cells > 1 ? cells : 1);
+  const auto key = std::make_pair(b->storage, key_size);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto                        it = g_tune.find(key);
+    if (it != g_tune.end())
+    {
+      b->chunk_rows = it->second.first;
+      b->prefetch = it->second.second;
+      if (chunk_rows_out) *chunk_rows_out = b->chunk_rows;
+      if (prefetch_out) *prefetch_out = b->prefetch;
+      return HMG_OK;
+    }
+  }
+  if (probe < 1) probe = 1;
+  static const int cand_rows[] = {16, 32, 48, 64, 96};
+  float            best = 1e30f;
+  int              best_rows = b->chunk_rows, best_pf = b->prefetch;
+  std::vector<int> step(b->maps.size());
+  for (int pf = 0; pf < 2; pf++)
+    for (int rows : cand_rows)
+    {
+      int min_left = 1 << 30;
+      for (int it : b->iters) min_left = it < min_left ? it : min_left;
+      if (min_left < probe) break; // keep every map's iteration count exact
+      for (size_t i = 0; i < step.size(); i++) step[i] = probe;
+      float ms = 0.f;
+      HMG_TRY(batch_advance(b->ctx, b->maps.data(), step.data(), (int)step.size(), 0, rows, pf,
+                            &ms));
+      for (size_t i = 0; i < step.size(); i++) b->iters[i] -= probe;
+      if (ms < best)
+      {
+        best = ms;
+        best_rows = rows;
+        best_pf = pf;
+      }
+    }
+  b->chunk_rows = best_rows;
+  b->prefetch = best_pf;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = std::make_pair(best_rows, best_pf);
+  }
+  if (chunk_rows_out) *chunk_rows_out = best_rows;
+  if (prefetch_out) *prefetch_out = best_pf;
+  return HMG_OK;
+}
+
+int hmg_batch_run(hmg_batch *b, int use_graph, float *device_ms)
+{
+  clear_error();
+  if (!b) return set_error(HMG_ERR_INVALID, "null batch");
+  std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
+  HMG_TRY(ctx_enter(b->ctx));
+  float ms = 0.f;
+  HMG_TRY(batch_advance(b->ctx, b->maps.data(), b->iters.data(), (int)b->maps.size(), use_graph,
+                        b->chunk_rows, b->prefetch, &ms));
+  for (int &it : b->iters) it = 0;
+  b->last_ms = ms;
+  if (device_ms) *device_ms = ms;
+  return HMG_OK;
+}
+
+int hmg_batch_export(hmg_batch *b, int map_id, float *z, float *d, float *s, float *u, float *v)
+{
+  clear_error();
+  if (!b || map_id < 0 || map_id >= (int)b->maps.size())
+    return set_error(HMG_ERR_INVALID, "bad batch or map id");
+  return hmg_session_export(b->maps[map_id], z, d, s, u, v);
+}
+
+int hmg_batch_destroy(hmg_batch *b)
+{
+  if (!b) return HMG_OK;
+  for (hmg_session *s : b->maps) hmg_session_destroy(s);
+  delete b;
+  return HMG_OK;
 }
 
 } // extern "C"

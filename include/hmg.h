@@ -265,6 +265,39 @@ int    hmg_session_destroy(hmg_session *s);
 /* Device bytes a session of this shape and storage holds. */
 size_t hmg_session_bytes(int nx, int ny, int storage);
 
+/* -------------------------------------------------------------- batch -- */
+
+/* A batch runs many independent maps (any shapes, each with its own
+ * iteration count) together: every launch advances all maps still running
+ * by one iteration, so small maps share the GPU instead of each paying a
+ * launch per pass.  Each map's result is identical to running it alone.
+ * No reference counterpart (the reference processes one Array per call);
+ * the per-map semantics are hmg_vpipes'. */
+typedef struct hmg_batch hmg_batch;
+int hmg_batch_create(hmg_ctx *ctx, int storage, const hmg_vpipes_params *p, hmg_batch **out);
+/* Adds a map (host arrays; rain may be NULL) to run `iterations`
+ * iterations; *map_id receives its index. */
+int hmg_batch_add(hmg_batch   *b,
+                  const float *z_host,
+                  int          nx,
+                  int          ny,
+                  const float *rain_host,
+                  int          iterations,
+                  int         *map_id);
+/* Launch shape of the batched kernel (rows per wavefront, load-ahead). */
+int hmg_batch_set_tuning(hmg_batch *b, int chunk_rows, int prefetch);
+/* Picks the launch shape by timing `probe` iterations of each candidate on
+ * the live maps (they count toward each map's iterations; results do not
+ * depend on the shape); caches the choice per storage type and batch size. */
+int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefetch_out);
+/* Runs every map to its iteration count.  use_graph captures the launch
+ * sequence in one hipGraph.  *device_ms (may be NULL) receives the elapsed
+ * device time of the launches. */
+int hmg_batch_run(hmg_batch *b, int use_graph, float *device_ms);
+/* Copies one map's state out as hmg_session_export does. */
+int hmg_batch_export(hmg_batch *b, int map_id, float *z, float *d, float *s, float *u, float *v);
+int hmg_batch_destroy(hmg_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,81 @@
+"""Batched launches: every map of a batch ends bit-identical to the same map
+run alone (and to the FP32 oracle), with or without hipGraph capture and for
+any launch shape the autotuner picks."""
+import numpy as np
+import pytest
+
+from conftest import VP_WORKLOAD
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, iterations, seed): mixed shapes (one rectangular 1:2, one tiny)
+MAPS = [(128, 128, 23, 100), (96, 192, 17, 101), (192, 192, 11, 102), (64, 64, 30, 103),
+        (257, 130, 9, 104), (5, 3, 14, 105), (300, 300, 6, 106)]
+
+
+def _single(hm, z0, iters, storage=0):
+    s = hm.Session(z0, storage=storage, **VP_WORKLOAD)
+    s.vpipes(iters)
+    out = s.export()
+    s.close()
+    return out
+
+
+def _assert_same(a, b):
+    for name, x, y in zip("zdsuv", a, b):
+        assert np.array_equal(x, y), "%s differs (max %g)" % (name, np.abs(x - y).max())
+
+
+def _inputs(orc):
+    return [orc.fbm_terrain(nx, ny, seed) if nx > 8 else
+            np.random.default_rng(seed).random((ny, nx), dtype=np.float32)
+            for nx, ny, _, seed in MAPS]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("storage", [0, 1])
+def test_batch_equals_single_sessions(hm, orc, use_graph, storage):
+    zs = _inputs(orc)
+    b = hm.Batch(storage=storage, **VP_WORKLOAD)
+    ids = [b.add(z, it) for z, (_, _, it, _) in zip(zs, MAPS)]
+    b.run(use_graph=use_graph)
+    for mid, z, (_, _, it, _) in zip(ids, zs, MAPS):
+        _assert_same(b.export(mid), _single(hm, z, it, storage))
+    b.close()
+
+
+def test_batch_against_oracle(hm, orc):
+    zs = _inputs(orc)
+    b = hm.Batch(**VP_WORKLOAD)
+    ids = [b.add(z, it) for z, (_, _, it, _) in zip(zs, MAPS)]
+    b.run()
+    for mid, z, (_, _, it, _) in zip(ids, zs, MAPS):
+        st = orc.VpState(z, orc.vp_params(**VP_WORKLOAD)).run(it)
+        _assert_same(b.export(mid), (st.z, st.d, st.s, st.u, st.v))
+    b.close()
+
+
+def test_autotune_keeps_results(hm, orc):
+    zs = _inputs(orc)
+    b = hm.Batch(**VP_WORKLOAD)
+    ids = [b.add(z, it + 30) for z, (_, _, it, _) in zip(zs, MAPS)]
+    rows, pf = b.autotune(probe=2)
+    assert rows >= 1 and pf in (0, 1)
+    b.run()
+    for mid, z, (_, _, it, _) in zip(ids, zs, MAPS):
+        _assert_same(b.export(mid), _single(hm, z, it + 30))
+    b.close()
+
+
+def test_batch_with_rain_map_and_zero_iterations(hm, orc):
+    z1 = orc.fbm_terrain(80, 60, 7)
+    z2 = orc.fbm_terrain(64, 64, 8)
+    rain = np.random.default_rng(2).random((60, 80), dtype=np.float32) + 0.5
+    b = hm.Batch(**VP_WORKLOAD)
+    a = b.add(z1, 12, rain_map=rain)
+    c = b.add(z2, 0)
+    b.run()
+    st = orc.VpState(z1, orc.vp_params(**VP_WORKLOAD), rain).run(12)
+    _assert_same(b.export(a), (st.z, st.d, st.s, st.u, st.v))
+    assert np.array_equal(b.export(c)[0], z2)
+    b.close()
