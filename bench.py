@@ -1,23 +1,26 @@
 #!/usr/bin/env python3
-"""Headline benchmark: vpipes + thermal on a 4096 x 4096 grid, one MI355X.
+"""Benchmarks of the HighMap grid-erosion hot path on one MI355X.
 
-One step = 10 hydraulic_vpipes iterations followed by one thermal step
-(talus 1/4096, then extrapolate_borders) on one continuous device-resident
-state, with the 1024^2-workload parameters (water_height 1e-3 with volume
-top-up, dt 0.5, k_capacity 0.1, k_erode 0.05, k_depose 0.05, evaporation
-0.01).  The default 50 timed steps are one full 500-iteration job.
+Default (the headline): vpipes + thermal on a 4096 x 4096 grid.  One step =
+10 hydraulic_vpipes iterations followed by one thermal step (talus 1/4096,
+then extrapolate_borders) on one continuous device-resident state, with the
+1024^2-workload parameters (water_height 1e-3 with volume top-up, dt 0.5,
+k_capacity 0.1, k_erode 0.05, k_depose 0.05, evaporation 0.01).  The default
+50 timed steps are one full 500-iteration job.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--storage f32|bf16|f16]
+    python bench.py --workload batch|serve|precision|thermal256|vpipes1024
 
-Prints one JSON line.  `value` is hydraulic cell-iterations per second
-(nx*ny*iterations / wall time of the timed steps, thermal included in the
-time), with the state already resident in HBM.  `roofline` is measured on the
-fused vpipes kernel alone with HIP events on the library's stream.
-`cpu_baseline` times the FP32 CPU restatement (oracle/cpu_bench) on a
-bounded sample of the same workload.
+Each run prints one JSON line.  For the headline, `value` is hydraulic
+cell-iterations per second (nx*ny*iterations / wall time of the timed steps,
+thermal included in the time) with the state already resident in HBM;
+`roofline` is measured on the fused vpipes kernel alone with HIP events on
+the library's stream; `cpu_baseline` times the FP32 CPU restatement
+(oracle/cpu_bench) on a bounded sample of the same workload.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -35,23 +38,31 @@ STORAGE = {"f32": 0, "bf16": 1, "f16": 2}
 ALG_BYTES = {"f32": 8 + 6 * 4 * 2, "bf16": 8 + 6 * 2 * 2, "f16": 8 + 6 * 2 * 2}
 
 
-def cpu_baseline(n, iters=20, reps=3):
+def cpu_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_bench(workload, nx, ny, iters, reps=3, threads=None):
     exe = os.path.join(ROOT, "oracle", "cpu_bench")
     if not os.path.exists(exe):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    out = subprocess.run([exe, "vpipes_thermal", str(n), str(n), str(iters), str(threads),
-                          str(reps)], capture_output=True, text=True, check=True).stdout
-    r = json.loads(out.strip().splitlines()[-1])
+    threads = threads or cpu_threads()
+    out = subprocess.run([exe, workload, str(nx), str(ny), str(iters), str(threads), str(reps)],
+                         capture_output=True, text=True, check=True).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def cpu_baseline(n, iters=20, reps=3):
+    r = cpu_bench("vpipes_thermal", n, n, iters, reps)
     return {
         "value": round(r["gcell_iter_s"], 5),
         "unit": "Gcell-iter/s",
-        "cores": threads,
+        "cores": r["threads"],
         "kind": "port",
         "sample": "%dx%d fBm seed 2, %d vpipes iterations with a thermal step every 10, "
                   "FP32 CPU restatement of the reference passes built -O3 -Ofast -ffast-math, "
-                  "OpenMP %d threads, median of %d runs after 1 warm-up (%.1f s each)"
-                  % (n, n, iters, threads, reps, r["median_s"]),
+                  "OpenMP %d threads, median of %d runs after 1 warm-up (%.2f s each)"
+                  % (n, n, iters, r["threads"], reps, r["median_s"]),
     }
 
 
@@ -66,42 +77,34 @@ def load_traffic(key):
         return None
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--storage", default="f32", choices=sorted(STORAGE))
-    ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
-    ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X and has no "
-              "multi-GPU path (requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world),
-              file=sys.stderr)
-        sys.exit(2)
-
+def _gpu():
     import torch
 
     import highmap_amd as hm
 
-    n = args.n
     hm.init_gpu(0)
-    ctx = hm._default_ctx
-    stream = torch.cuda.ExternalStream(ctx.stream)
-    dev = torch.device("cuda", 0)
+    return torch, hm, hm._default_ctx
 
-    # input: remap(noise_fbm(PERLIN, kw 4, seed 2)) generated on the device
-    z = torch.empty((n, n), dtype=torch.float32, device=dev)
-    L = hm.lib()
-    hm._check(L.hmg_noise_fbm_dev(ctx.handle, z.data_ptr(), n, n, hm.NoiseType.PERLIN, 4.0, 4.0,
-                                  2, 8, 0.7, 0.5, 2.0, None, None, None, None, None))
-    hm.remap(z.data_ptr(), n * n, 0.0, 1.0)
+
+def device_terrain(torch, hm, ctx, nx, ny, seed):
+    """remap(noise_fbm(PERLIN, kw 4, seed)) to [0,1], generated on the GPU."""
+    z = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    hm._check(hm.lib().hmg_noise_fbm_dev(ctx.handle, z.data_ptr(), nx, ny, hm.NoiseType.PERLIN,
+                                         4.0, 4.0, seed, 8, 0.7, 0.5, 2.0, None, None, None,
+                                         None, None))
+    hm.remap(z.data_ptr(), nx * ny, 0.0, 1.0)
     torch.cuda.synchronize()
+    return z
 
+
+# ------------------------------------------------------------- headline
+
+
+def run_headline(args):
+    torch, hm, ctx = _gpu()
+    n = args.n
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    z = device_terrain(torch, hm, ctx, n, n, 2)
     sess = hm.Session(z, storage=STORAGE[args.storage], mode=args.mode, **WORKLOAD)
     talus = 1.0 / n
 
@@ -125,8 +128,7 @@ def main():
     sess.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = e0.elapsed_time(e1)
-    elapsed = max(wall, gpu_ms / 1e3)
+    elapsed = max(wall, e0.elapsed_time(e1) / 1e3)
     iters = 10 * args.steps
     value = n * n * iters / elapsed / 1e9
 
@@ -141,8 +143,7 @@ def main():
     achieved = alg / launch_s / 1e9
     sess.close()
 
-    kernel = "k_vp_fused" if args.mode == "fused" else "k_vp_* (plain passes)"
-    traffic = load_traffic("vpipes_%s_%s_%d" % (args.mode, args.storage, n))
+    kernel = "k_vp_stream" if args.mode == "fused" else "k_vp_* (plain passes)"
     out = {
         "metric": "Gcell-iter/s vpipes+thermal 4096^2",
         "value": round(value, 3),
@@ -176,13 +177,262 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
+            "traffic": load_traffic("vpipes_%s_%s_%d" % (args.mode, args.storage, n)),
             "alg_bytes_per_launch": alg,
             "avg_launch_us": round(launch_s * 1e6, 2),
         },
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)
+    return out
+
+
+# ---------------------------------------------------------------- batch
+
+
+def run_batch(args):
+    """96 maps (seeds 100..195, shapes and iteration counts from the batch
+    manifest): one batched launch per iteration captured in a hipGraph, with
+    the autotuned launch shape, against the same maps run one after another
+    with per-map plain launches (and per-map fused launches)."""
+    from highmap_amd import workloads
+
+    torch, hm, ctx = _gpu()
+    maps = workloads.batch_manifest()
+    if args.maps:
+        maps = maps[: args.maps]
+    zs = [device_terrain(torch, hm, ctx, m["nx"], m["ny"], m["seed"]).cpu().numpy()
+          for m in maps]
+    cell_iters = sum(m["nx"] * m["ny"] * m["iterations"] for m in maps)
+    st = STORAGE[args.storage]
+
+    # warm-up and autotune on a throw-away copy
+    b = hm.Batch(storage=st, **WORKLOAD)
+    for z, m in zip(zs, maps):
+        b.add(z, 24)
+    tuned = b.autotune(probe=2)
+    b.close()
+
+    b = hm.Batch(storage=st, **WORKLOAD)
+    for z, m in zip(zs, maps):
+        b.add(z, m["iterations"])
+    b.set_tuning(*tuned)
+    t0 = time.perf_counter()
+    dev_ms = b.run(use_graph=True)
+    batched_s = time.perf_counter() - t0
+    b.close()
+
+    def per_map(mode):
+        sess = []
+        for z in zs:
+            sess.append(hm.Session(z, storage=st, mode=mode, **WORKLOAD))
+        ctx.sync()
+        t0 = time.perf_counter()
+        for s, m in zip(sess, maps):
+            s.vpipes(m["iterations"])
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        for s in sess:
+            s.close()
+        return dt
+
+    plain_s = per_map("plain")
+    fused_s = per_map("fused")
+    return {
+        "metric": "Gcell-iter/s batched vpipes, 96 uneven maps",
+        "value": round(cell_iters / batched_s / 1e9, 3),
+        "unit": "Gcell-iter/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32" if args.storage == "f32" else "f32 math, %s storage" % args.storage,
+        "data": "synthetic: remap(noise_fbm PERLIN, kw 4, seed s) per map, s = 100..195",
+        "config": {"workload": "batch manifest: %d maps, %d cells, %d cell-iterations"
+                               % (len(maps), sum(m["nx"] * m["ny"] for m in maps), cell_iters),
+                   "params": WORKLOAD, "tuning": {"chunk_rows": tuned[0], "prefetch": tuned[1]}},
+        "batched_graph_s": round(batched_s, 4),
+        "batched_device_ms": round(dev_ms, 2),
+        "per_map_plain_s": round(plain_s, 4),
+        "per_map_plain_gcell_iter_s": round(cell_iters / plain_s / 1e9, 3),
+        "per_map_fused_s": round(fused_s, 4),
+        "per_map_fused_gcell_iter_s": round(cell_iters / fused_s / 1e9, 3),
+        "speedup_vs_per_map_plain": round(plain_s / batched_s, 2),
+    }
+
+
+# ------------------------------------------------------------ precision
+
+
+def run_precision(args):
+    """FP32 / BF16 / FP16 storage of water, sediment and fluxes against the
+    FP64 CPU oracle on one continuous vpipes+thermal state (thermal every 10
+    iterations): max |z - z64| / range(z64) and the drift of sum(z+s) at
+    checkpoints, for the workload parameters and the reference defaults."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+
+    torch, hm, ctx = _gpu()
+    n, total = args.n, args.iters
+    checkpoints = [c for c in (10, 20, 50, 100, 200, 500, 1000) if c < total] + [total]
+    z0 = device_terrain(torch, hm, ctx, n, n, 2).cpu().numpy()
+    mass0 = float(z0.astype(np.float64).sum())
+    out = {"metric": "max |z - z_fp64| / range and sum(z+s) drift vs the FP64 oracle",
+           "unit": "relative", "n_gpus": 1, "config": {"nx": n, "ny": n, "iterations": total,
+                                                        "thermal_every": 10},
+           "results": {}}
+    for pname, params in (("workload", WORKLOAD), ("reference_defaults", {})):
+        o = O.VpState(z0.astype(np.float64), O.vp_params(**params), dtype=np.float64)
+        sess = {k: hm.Session(z0, storage=v, **params) for k, v in STORAGE.items()}
+        done = 0
+        rows = []
+        t0 = time.perf_counter()
+        for cp in checkpoints:
+            while done < cp:
+                step = min(10 - done % 10, cp - done)
+                o.run(step)
+                for s in sess.values():
+                    s.vpipes(step)
+                done += step
+                if done % 10 == 0:
+                    o.thermal_step(1.0 / n, 1)
+                    for s in sess.values():
+                        s.thermal(1.0 / n, 1)
+            rng = float(o.z.max() - o.z.min())
+            m64 = float((o.z + o.s).sum())
+            row = {"iteration": cp, "fp64_mass_drift": (m64 - mass0) / mass0}
+            for k, s in sess.items():
+                z, d, sed, u, v = s.export()
+                row[k] = {"max_abs_err_rel_range": float(np.abs(z - o.z).max() / rng),
+                          "mass_vs_fp64_rel": float(((z.astype(np.float64) + sed).sum() - m64)
+                                                    / m64)}
+            rows.append(row)
+        for s in sess.values():
+            s.close()
+        out["results"][pname] = {"params": params or "hmap::gpu::hydraulic_vpipes defaults",
+                                 "checkpoints": rows,
+                                 "oracle_s": round(time.perf_counter() - t0, 1)}
+    return out
+
+
+# ------------------------------------------------ configs 1 and 2 (drop-in)
+
+
+def run_thermal256(args):
+    """hmap::gpu::thermal drop-in on 256^2 fBm (seed 1), talus 2/256, 50
+    iterations: host-pointer call (transfers included) and device-resident,
+    against the CPU restatement (1 thread and all threads) and the FP64 oracle."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+
+    torch, hm, ctx = _gpu()
+    z0 = O.fbm_terrain(256, 256, 1)
+    z = z0.copy()
+    hm.thermal(z, 2.0 / 256, 50)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        z = z0.copy()
+        hm.thermal(z, 2.0 / 256, 50)
+    call_s = (time.perf_counter() - t0) / reps
+    zd = torch.from_numpy(z0).cuda()
+    hm._check(hm.lib().hmg_thermal_dev(ctx.handle, zd.data_ptr(), 256, 256, None, 2.0 / 256, 50,
+                                       None))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hm._check(hm.lib().hmg_thermal_dev(ctx.handle, zd.data_ptr(), 256, 256, None, 2.0 / 256,
+                                           50, None))
+    dev_s = (time.perf_counter() - t0) / reps
+    o64 = O.thermal(z0.astype(np.float64), 2.0 / 256, 50, np.float64)
+    c1 = cpu_bench("thermal", 256, 256, 50, 5, 1)
+    cn = cpu_bench("thermal", 256, 256, 50, 5)
+    cells = 256 * 256 * 50
+    return {"metric": "Gcell-iter/s thermal 256^2 x 50 (drop-in)",
+            "value": round(cells / call_s / 1e9, 4), "unit": "Gcell-iter/s", "n_gpus": 1,
+            "device_resident_gcell_iter_s": round(cells / dev_s / 1e9, 4),
+            "bitexact_vs_fp32_oracle": bool(np.array_equal(z, O.thermal(z0, 2.0 / 256, 50))),
+            "max_abs_err_rel_range_vs_fp64": float(np.abs(z - o64).max() / (o64.max() - o64.min())),
+            "cpu_1_thread_gcell_iter_s": c1["gcell_iter_s"],
+            "cpu_all_threads_gcell_iter_s": cn["gcell_iter_s"], "cpu_threads": cn["threads"]}
+
+
+def run_vpipes1024(args):
+    """hydraulic_vpipes at 1024^2 (seed 2), 200 iterations: (i) reference
+    semantics (gravity 1, volume top-up) with plain and fused launches,
+    (ii) gravity 9.81 with 1e-3 rain per iteration; GPU vs CPU restatement,
+    error vs the FP64 oracle."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+
+    torch, hm, ctx = _gpu()
+    n, iters = 1024, 200
+    z0 = O.fbm_terrain(n, n, 2)
+    res = {}
+    for name, extra in (("i_reference", {}),
+                        ("ii_gravity_rain", dict(gravity=9.81, rain_per_iter=1e-3,
+                                                 maintain_water_volume=0))):
+        kw = dict(WORKLOAD)
+        kw.update(extra)
+        row = {}
+        for mode in ("plain", "fused"):
+            z = z0.copy()
+            hm.hydraulic_vpipes(z, iterations=5, mode=mode, **kw)  # warm-up
+            t0 = time.perf_counter()
+            z = z0.copy()
+            hm.hydraulic_vpipes(z, iterations=iters, mode=mode, **kw)
+            row[mode + "_dropin_gcell_iter_s"] = round(n * n * iters / (time.perf_counter() - t0)
+                                                       / 1e9, 3)
+            s = hm.Session(z0, mode=mode, **kw)
+            s.vpipes(2)
+            s.sync()
+            t0 = time.perf_counter()
+            s.vpipes(iters)
+            s.sync()
+            row[mode + "_resident_gcell_iter_s"] = round(n * n * iters / (time.perf_counter() - t0)
+                                                         / 1e9, 3)
+            s.close()
+        o64 = O.VpState(z0.astype(np.float64), O.vp_params(**kw), dtype=np.float64).run(iters)
+        row["max_abs_err_rel_range_vs_fp64"] = float(np.abs(z - o64.z).max()
+                                                     / (o64.z.max() - o64.z.min()))
+        res[name] = row
+    c = cpu_bench("vpipes", n, n, 20, 3)
+    return {"metric": "Gcell-iter/s vpipes 1024^2 x 200", "unit": "Gcell-iter/s", "n_gpus": 1,
+            "value": res["i_reference"]["fused_resident_gcell_iter_s"], "results": res,
+            "cpu_all_threads_gcell_iter_s": c["gcell_iter_s"], "cpu_threads": c["threads"]}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--storage", default="f32", choices=sorted(STORAGE))
+    ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--iters", type=int, default=500)
+    ap.add_argument("--maps", type=int, default=0)
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "batch", "serve", "precision", "thermal256",
+                             "vpipes1024"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X and has no "
+              "multi-GPU path (requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world),
+              file=sys.stderr)
+        sys.exit(2)
+    fn = {"headline": run_headline, "batch": run_batch, "precision": run_precision,
+          "thermal256": run_thermal256, "vpipes1024": run_vpipes1024}
+    if args.workload == "serve":
+        from highmap_amd import serve
+
+        out = serve.bench(args)
+    else:
+        out = fn[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

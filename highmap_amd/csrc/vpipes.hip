@@ -917,9 +917,17 @@ static int session_create_common(hmg_ctx                 *ctx,
   s->p = prm;
   s->k = make_vp_const(prm, nx, ny);
   s->esize = storage == HMG_STORAGE_F32 ? 4 : 2;
-  // measured best launch shapes at 4096^2 (fused kernel): 64 rows per
-  // wavefront for FP32 state, 48 for 16-bit state; loads one row ahead
-  s->chunk_rows = storage == HMG_STORAGE_F32 ? 64 : 48;
+  // Launch shape of the fused kernel: at 4096^2 the measured best is 64 rows
+  // per wavefront for FP32 state and 48 for 16-bit state (loads one row
+  // ahead).  Smaller grids get shorter chunks so that a launch still has
+  // about 16 wavefronts per CU (never fewer than 8 rows per wavefront).
+  {
+    const long strips = (nx + STRIP_OUT - 1) / STRIP_OUT;
+    const long target = 16L * (ctx->num_cus > 0 ? ctx->num_cus : 256);
+    long       ch = (ny * strips + target - 1) / target;
+    const long cap = storage == HMG_STORAGE_F32 ? 64 : 48;
+    s->chunk_rows = (int)(ch < 8 ? 8 : (ch > cap ? cap : ch));
+  }
   s->prefetch = 1;
   const size_t n = (size_t)nx * ny;
   int          rc = HMG_OK;
