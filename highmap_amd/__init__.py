@@ -103,6 +103,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise HmgError(-3, "%s is missing; build it with `python -m highmap_amd.build`"
                            % LIB_PATH)
+        # One HIP runtime per process: torch ships its own libamdhip64 with the
+        # same soname; loading torch first makes libhmg bind to that copy, so
+        # torch tensors and libhmg share the runtime (a second runtime loaded
+        # after ours would find no GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, i32, f32, u32, sz = C.c_void_p, C.c_int, C.c_float, C.c_uint32, C.c_size_t
         pp = C.POINTER(VpipesParams)

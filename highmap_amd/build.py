@@ -25,8 +25,7 @@ LIB = os.path.join(HERE, "libhmg.so")
 SHIM_LIB = os.path.join(HERE, "libhmap_gpu.so")
 ARCH = os.environ.get("HMG_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["runtime.hip", "noise.hip", "thermal.hip", "vpipes.hip", "batch.hip",
-               "server.cpp"]
+HIP_SOURCES = ["runtime.hip", "noise.hip", "thermal.hip", "vpipes.hip", "server.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=" + ARCH,

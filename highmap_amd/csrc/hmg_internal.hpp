@@ -132,6 +132,42 @@ int launch_thermal(hmg_ctx     *ctx,
                    const float *bedrock,
                    hipStream_t  st);
 int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
+
+} // namespace hmg
+
+// One erosion state resident on the device (see vpipes.hip).
+struct hmg_session
+{
+  hmg_ctx          *ctx = nullptr;
+  int               nx = 0, ny = 0;
+  int               storage = HMG_STORAGE_F32;
+  hmg_vpipes_params p{};
+  hmg::VpConst      k{};
+  size_t            esize = 4; // bytes per stored scalar
+  hmg::DevBuf       z[2], D[2], Sp[2], F[2], rain, slots;
+  hmg::DevBuf       d1, s, u, v; // plain-form intermediates
+  hmg::DevBuf       zscratch;    // thermal ping-pong
+  int               cur = 0;
+  long              iter = 0;
+  double            W0 = 0.0, R = 0.0;
+  float             rf = 0.f;
+  int               chunk_rows = 64;
+  int               prefetch = 1;
+};
+
+namespace hmg
+{
+
+// Advances sessions[i] by iters[i] hydraulic iterations with batched
+// launches (vpipes.hip).  *ms (may be null) receives the device time.
+int batch_advance(hmg_ctx           *ctx,
+                  hmg_session *const *ss,
+                  const int          *iters,
+                  int                 n,
+                  int                 use_graph,
+                  int                 chunk_rows,
+                  int                 prefetch,
+                  float              *ms);
 int launch_noise_fbm(float       *out,
                      int          nx,
                      int          ny,

@@ -1,0 +1,390 @@
+// server.hip -- continuous-batching erosion server on one device.
+//
+// Requests (erosion jobs of any grid size and iteration count) are queued on
+// submit and admitted in arrival order while their device state fits in an
+// HBM budget.  Admitted jobs keep their state resident in device memory and
+// advance together: each tick runs up to 10 hydraulic iterations of every
+// vpipes job as batched launches (one launch per iteration for all jobs),
+// the thermal step those jobs are due, and up to 8 Jacobi steps of every
+// thermal-only job.  Jobs join at the next tick after they are admitted and
+// leave as soon as they finish, when their result is copied to the caller's
+// host buffer and their memory returns to the budget.
+//
+// A job's result depends only on its own state: it is bit-identical to the
+// same job run alone (hmg_session_* / hmg_thermal_dev), whatever else was
+// running.  No reference counterpart: HighMap runs one call at a time.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+
+#include "hmg_internal.hpp"
+
+namespace
+{
+
+double now_s()
+{
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Job
+{
+  long long          id = 0;
+  hmg_job_desc       desc{};
+  std::vector<float> z_in; // host copy when the caller passed z
+  float             *out = nullptr;
+  double             t_submit = 0, t_admit = 0, t_done = 0;
+  int                state = HMG_JOB_QUEUED;
+  hmg_session       *s = nullptr;
+  int                done = 0; // iterations (vpipes) or steps (thermal) done
+  size_t             bytes = 0;
+};
+
+} // namespace
+
+struct hmg_server
+{
+  hmg_ctx                      *ctx = nullptr;
+  int                           storage = HMG_STORAGE_F32;
+  hmg_vpipes_params             p{};
+  size_t                        budget = 0, used = 0;
+  std::mutex                    mu;
+  std::condition_variable       cv, cv_done;
+  std::deque<Job *>             queue;
+  std::vector<Job *>            active;
+  std::map<long long, Job *>    jobs;
+  std::thread                   worker;
+  bool                          stop = false;
+  long long                     next_id = 1;
+  double                        cell_iters = 0, busy_s = 0;
+  long long                     ticks = 0;
+};
+
+namespace
+{
+
+using namespace hmg;
+
+size_t job_bytes(const hmg_server *sv, const hmg_job_desc &d)
+{
+  const size_t n = (size_t)d.nx * (size_t)d.ny;
+  const size_t g = (size_t)1 << 16; // pool granularity
+  size_t       b = hmg_session_bytes(d.nx, d.ny, sv->storage) + 12 * g;
+  if (!d.has_input) b += n * 4 + g; // the generated input terrain
+  return b;
+}
+
+int admit(hmg_server *sv, Job *j)
+{
+  hmg_ctx           *ctx = sv->ctx;
+  const hmg_job_desc &d = j->desc;
+  hmg_vpipes_params  p = sv->p;
+  p.mode = HMG_VP_FUSED;
+  if (d.has_input)
+    return hmg_session_create(ctx, j->z_in.data(), d.nx, d.ny, &p, nullptr, sv->storage, &j->s);
+  // generate remap(noise_fbm(PERLIN, kw 4, seed)) on the device
+  DevBuf t;
+  HMG_TRY(t.get(ctx->pool, (size_t)d.nx * d.ny * sizeof(float)));
+  HMG_TRY(hmg_noise_fbm_dev(ctx, t.as<float>(), d.nx, d.ny, 1, 4.f, 4.f, d.seed, 8, 0.7f, 0.5f,
+                            2.f, nullptr, nullptr, nullptr, nullptr, nullptr));
+  HMG_TRY(hmg_remap_dev(ctx, t.as<float>(), (size_t)d.nx * d.ny, 0.f, 1.f));
+  return hmg_session_create_dev(ctx, t.as<float>(), d.nx, d.ny, &p, nullptr, sv->storage, &j->s);
+}
+
+// One tick over the active jobs; returns the cell-iterations done.
+double tick(hmg_server *sv, std::vector<Job *> &act)
+{
+  hmg_ctx                    *ctx = sv->ctx;
+  std::vector<hmg_session *> vs;
+  std::vector<int>           vi;
+  std::vector<Job *>         vj;
+  double                     work = 0;
+  for (Job *j : act)
+    if (j->desc.kind == HMG_JOB_VPIPES_THERMAL)
+    {
+      const int every = j->desc.thermal_every > 0 ? j->desc.thermal_every : (1 << 30);
+      int       k = every - j->done % every;
+      if (k > j->desc.iterations - j->done) k = j->desc.iterations - j->done;
+      if (k > 10) k = 10;
+      vs.push_back(j->s);
+      vi.push_back(k);
+      vj.push_back(j);
+    }
+  int rc = HMG_OK;
+  if (!vs.empty())
+  {
+    rc = batch_advance(ctx, vs.data(), vi.data(), (int)vs.size(), 0, vs[0]->chunk_rows, 1,
+                       nullptr);
+    for (size_t q = 0; q < vj.size(); q++)
+    {
+      Job *j = vj[q];
+      if (rc != HMG_OK)
+      {
+        j->state = rc;
+        continue;
+      }
+      j->done += vi[q];
+      work += (double)j->desc.nx * j->desc.ny * vi[q];
+      const int every = j->desc.thermal_every;
+      if (every > 0 && j->done % every == 0)
+      {
+        int r2 = hmg_session_thermal(j->s, j->desc.talus, 1);
+        if (r2 != HMG_OK) j->state = r2;
+      }
+    }
+  }
+  for (Job *j : act)
+    if (j->desc.kind == HMG_JOB_THERMAL)
+    {
+      int k = j->desc.iterations - j->done;
+      if (k > 8) k = 8;
+      hmg_session *s = j->s;
+      int          r2 = launch_thermal(ctx, s->z[s->cur].as<float>(), s->z[s->cur ^ 1].as<float>(),
+                                       s->nx, s->ny, nullptr, j->desc.talus, k, nullptr, ctx->stream);
+      if (r2 != HMG_OK)
+      {
+        j->state = r2;
+        continue;
+      }
+      j->done += k;
+      work += (double)j->desc.nx * j->desc.ny * k;
+      if (j->done == j->desc.iterations)
+      {
+        r2 = launch_extrapolate_borders(s->z[s->cur].as<float>(), s->nx, s->ny, ctx->stream);
+        if (r2 != HMG_OK) j->state = r2;
+      }
+    }
+  (void)hipStreamSynchronize(ctx->stream);
+  return work;
+}
+
+void worker_main(hmg_server *sv)
+{
+  for (;;)
+  {
+    std::vector<Job *> admitted;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv.wait(lk, [&] { return sv->stop || !sv->queue.empty() || !sv->active.empty(); });
+      if (sv->stop && sv->queue.empty() && sv->active.empty()) return;
+      // admission in arrival order while the state fits in the budget
+      while (!sv->queue.empty())
+      {
+        Job *j = sv->queue.front();
+        if (sv->used + j->bytes > sv->budget && !(sv->active.empty() && admitted.empty()))
+          break;
+        sv->queue.pop_front();
+        sv->used += j->bytes;
+        admitted.push_back(j);
+      }
+    }
+    const double t0 = now_s();
+    std::lock_guard<std::recursive_mutex> ctx_lock(sv->ctx->mu);
+    (void)ctx_enter(sv->ctx);
+    for (Job *j : admitted)
+    {
+      if (j->bytes > sv->budget)
+      {
+        j->state = HMG_ERR_OOM;
+        continue;
+      }
+      int rc = admit(sv, j);
+      j->t_admit = now_s();
+      j->state = rc == HMG_OK ? HMG_JOB_RUNNING : rc;
+    }
+    std::vector<Job *> act;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      for (Job *j : admitted) sv->active.push_back(j);
+      for (Job *j : sv->active)
+        if (j->state == HMG_JOB_RUNNING && j->done < j->desc.iterations) act.push_back(j);
+    }
+    const double work = act.empty() ? 0.0 : tick(sv, act);
+    // results of finished (or failed) jobs to the callers' host buffers
+    std::vector<Job *> finished;
+    for (Job *j : sv->active)
+      if (j->state != HMG_JOB_RUNNING || j->done >= j->desc.iterations) finished.push_back(j);
+    for (Job *j : finished)
+    {
+      if (j->state == HMG_JOB_RUNNING && j->s)
+      {
+        int rc = HMG_OK;
+        // thermal() extrapolates the borders even after zero steps
+        if (j->desc.kind == HMG_JOB_THERMAL && j->desc.iterations == 0)
+          rc = launch_extrapolate_borders(j->s->z[j->s->cur].as<float>(), j->s->nx, j->s->ny,
+                                          sv->ctx->stream);
+        if (rc == HMG_OK)
+          rc = hmg_session_export(j->s, j->out, nullptr, nullptr, nullptr, nullptr);
+        j->state = rc == HMG_OK ? HMG_JOB_DONE : rc;
+      }
+      if (j->s) hmg_session_destroy(j->s);
+      j->s = nullptr;
+      j->t_done = now_s();
+    }
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      for (Job *j : finished)
+      {
+        sv->used -= j->bytes;
+        sv->active.erase(std::find(sv->active.begin(), sv->active.end(), j));
+      }
+      sv->cell_iters += work;
+      sv->busy_s += now_s() - t0;
+      sv->ticks++;
+    }
+    if (!finished.empty()) sv->cv_done.notify_all();
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int hmg_server_create(hmg_ctx                 *ctx,
+                      size_t                   hbm_budget,
+                      int                      storage,
+                      const hmg_vpipes_params *p,
+                      hmg_server             **out)
+{
+  clear_error();
+  if (!ctx || !out) return set_error(HMG_ERR_INVALID, "null argument");
+  if (storage != HMG_STORAGE_F32 && storage != HMG_STORAGE_BF16 && storage != HMG_STORAGE_F16)
+    return set_error(HMG_ERR_INVALID, "unknown storage %d", storage);
+  hmg_server *sv = new hmg_server();
+  sv->ctx = ctx;
+  sv->storage = storage;
+  if (p)
+    sv->p = *p;
+  else
+    hmg_vpipes_defaults(&sv->p);
+  int rc = validate_vp_params(sv->p);
+  if (rc != HMG_OK)
+  {
+    delete sv;
+    return rc;
+  }
+  if (hbm_budget == 0)
+  {
+    size_t free_b = 0, total_b = 0;
+    (void)ctx_enter(ctx);
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+    hbm_budget = free_b / 10 * 8;
+  }
+  sv->budget = hbm_budget;
+  sv->worker = std::thread(worker_main, sv);
+  *out = sv;
+  return HMG_OK;
+}
+
+int hmg_server_submit(hmg_server         *sv,
+                      const hmg_job_desc *d,
+                      const float        *z_host,
+                      float              *z_out_host,
+                      long long          *job_id)
+{
+  clear_error();
+  if (!sv || !d || !z_out_host) return set_error(HMG_ERR_INVALID, "null argument");
+  if (d->nx < 3 || d->ny < 3 || (size_t)d->nx * (size_t)d->ny > ((size_t)1 << 31))
+    return set_error(HMG_ERR_INVALID, "job grid must be at least 3x3 and below 2^31 cells");
+  if (d->iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
+  if (d->kind != HMG_JOB_VPIPES_THERMAL && d->kind != HMG_JOB_THERMAL)
+    return set_error(HMG_ERR_INVALID, "unknown job kind %d", d->kind);
+  if ((d->has_input != 0) != (z_host != nullptr))
+    return set_error(HMG_ERR_INVALID, "has_input must match z_host");
+  Job *j = new Job();
+  j->desc = *d;
+  j->out = z_out_host;
+  if (z_host) j->z_in.assign(z_host, z_host + (size_t)d->nx * d->ny);
+  j->bytes = job_bytes(sv, *d);
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop)
+    {
+      delete j;
+      return set_error(HMG_ERR_INVALID, "server is stopping");
+    }
+    j->id = sv->next_id++;
+    j->t_submit = now_s();
+    sv->jobs[j->id] = j;
+    sv->queue.push_back(j);
+  }
+  sv->cv.notify_one();
+  if (job_id) *job_id = j->id;
+  return HMG_OK;
+}
+
+int hmg_server_poll(hmg_server *sv, long long job_id, int *state)
+{
+  if (!sv || !state) return set_error(HMG_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto                        it = sv->jobs.find(job_id);
+  if (it == sv->jobs.end()) return set_error(HMG_ERR_INVALID, "unknown job %lld", job_id);
+  *state = it->second->state;
+  return HMG_OK;
+}
+
+int hmg_server_job_times(hmg_server *sv,
+                         long long   job_id,
+                         double     *t_submit,
+                         double     *t_admit,
+                         double     *t_done)
+{
+  if (!sv) return set_error(HMG_ERR_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto                        it = sv->jobs.find(job_id);
+  if (it == sv->jobs.end()) return set_error(HMG_ERR_INVALID, "unknown job %lld", job_id);
+  if (t_submit) *t_submit = it->second->t_submit;
+  if (t_admit) *t_admit = it->second->t_admit;
+  if (t_done) *t_done = it->second->t_done;
+  return HMG_OK;
+}
+
+int hmg_server_wait_all(hmg_server *sv)
+{
+  if (!sv) return set_error(HMG_ERR_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  sv->cv_done.wait(lk, [&] { return sv->queue.empty() && sv->active.empty(); });
+  return HMG_OK;
+}
+
+int hmg_server_stats(hmg_server *sv,
+                     double     *cell_iters,
+                     long long  *ticks,
+                     double     *busy_s,
+                     size_t     *budget,
+                     size_t     *used)
+{
+  if (!sv) return set_error(HMG_ERR_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (cell_iters) *cell_iters = sv->cell_iters;
+  if (ticks) *ticks = sv->ticks;
+  if (busy_s) *busy_s = sv->busy_s;
+  if (budget) *budget = sv->budget;
+  if (used) *used = sv->used;
+  return HMG_OK;
+}
+
+int hmg_server_destroy(hmg_server *sv)
+{
+  if (!sv) return HMG_OK;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (auto &kv : sv->jobs)
+  {
+    if (kv.second->s) hmg_session_destroy(kv.second->s);
+    delete kv.second;
+  }
+  delete sv;
+  return HMG_OK;
+}
+
+} // extern "C"

@@ -658,25 +658,6 @@ __global__ void k_zero_fix(Fix128 *p, int n)
 
 using namespace hmg;
 
-struct hmg_session
-{
-  hmg_ctx          *ctx = nullptr;
-  int               nx = 0, ny = 0;
-  int               storage = HMG_STORAGE_F32;
-  hmg_vpipes_params p{};
-  VpConst           k{};
-  size_t            esize = 4; // bytes per stored scalar
-  DevBuf            z[2], D[2], Sp[2], F[2], rain, slots;
-  DevBuf            d1, s, u, v; // plain-form intermediates
-  DevBuf            zscratch;    // thermal ping-pong
-  int               cur = 0;
-  long              iter = 0;
-  double            W0 = 0.0, R = 0.0;
-  float             rf = 0.f;
-  int               chunk_rows = 64;
-  int               prefetch = 1;
-};
-
 namespace hmg
 {
 

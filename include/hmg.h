@@ -298,6 +298,71 @@ int hmg_batch_run(hmg_batch *b, int use_graph, float *device_ms);
 int hmg_batch_export(hmg_batch *b, int map_id, float *z, float *d, float *s, float *u, float *v);
 int hmg_batch_destroy(hmg_batch *b);
 
+/* ------------------------------------------------------------- server -- */
+
+/* A continuous-batching erosion server on one device: jobs are admitted in
+ * arrival order while their state fits in an HBM budget, keep their state
+ * resident on the device, and advance together (batched vpipes launches,
+ * then due thermal steps) tick by tick; each finished job's z is copied to
+ * the caller's buffer.  A job's result equals the same job run alone.
+ * No reference counterpart (HighMap runs one call at a time). */
+typedef struct hmg_server hmg_server;
+
+enum
+{
+  HMG_JOB_VPIPES_THERMAL = 0, /* hydraulic iterations, a thermal step (1 Jacobi
+                                 step + borders) after every thermal_every-th */
+  HMG_JOB_THERMAL = 1         /* thermal(z, talus, iterations) only           */
+};
+enum
+{
+  HMG_JOB_QUEUED = 1,
+  HMG_JOB_RUNNING = 2,
+  HMG_JOB_DONE = 3 /* a negative state is the hmg_status of a failed job */
+};
+
+typedef struct hmg_job_desc
+{
+  int      nx, ny;
+  int      kind;          /* HMG_JOB_*                                      */
+  int      iterations;    /* hydraulic iterations, or thermal steps          */
+  int      thermal_every; /* HMG_JOB_VPIPES_THERMAL: 10 in the served mix    */
+  float    talus;         /* thermal talus                                   */
+  int      has_input;     /* 1: z given at submit; 0: generated on the device
+                             as remap(noise_fbm(PERLIN, kw 4, seed)) in [0,1] */
+  uint32_t seed;
+} hmg_job_desc;
+
+/* hbm_budget 0 means 80% of the device memory free at creation. */
+int hmg_server_create(hmg_ctx                 *ctx,
+                      size_t                   hbm_budget,
+                      int                      storage,
+                      const hmg_vpipes_params *p,
+                      hmg_server             **out);
+/* Queues a job; z_out_host (nx*ny floats) must stay valid until the job is
+ * done.  Returns immediately. */
+int hmg_server_submit(hmg_server         *sv,
+                      const hmg_job_desc *d,
+                      const float        *z_host,
+                      float              *z_out_host,
+                      long long          *job_id);
+int hmg_server_poll(hmg_server *sv, long long job_id, int *state);
+/* steady-clock seconds of submission, admission and result delivery */
+int hmg_server_job_times(hmg_server *sv,
+                         long long   job_id,
+                         double     *t_submit,
+                         double     *t_admit,
+                         double     *t_done);
+int hmg_server_wait_all(hmg_server *sv);
+int hmg_server_stats(hmg_server *sv,
+                     double     *cell_iters,
+                     long long  *ticks,
+                     double     *busy_s,
+                     size_t     *budget,
+                     size_t     *used);
+/* Finishes every queued job, then stops the server. */
+int hmg_server_destroy(hmg_server *sv);
+
 #ifdef __cplusplus
 }
 #endif
