@@ -9,7 +9,7 @@ k_capacity 0.1, k_erode 0.05, k_depose 0.05, evaporation 0.01).  The default
 50 timed steps are one full 500-iteration job.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--storage f32|bf16|f16]
-    python bench.py --workload batch|serve|precision|thermal256|vpipes1024
+    python bench.py --workload batch|serve|precision|thermal256|vpipes1024|copy
 
 Each run prints one JSON line.  For the headline, `value` is hydraulic
 cell-iterations per second (nx*ny*iterations / wall time of the timed steps,
@@ -402,6 +402,25 @@ def run_vpipes1024(args):
             "cpu_all_threads_gcell_iter_s": c["gcell_iter_s"], "cpu_threads": c["threads"]}
 
 
+def run_copy(args):
+    """A plain device copy of known size (1 GiB, three times, caches flushed
+    by a 2 GiB fill in between), used to calibrate the HBM byte counters."""
+    import torch
+
+    n = 256 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    flush = torch.empty(2 * n, dtype=torch.float32, device="cuda")
+    t0 = time.perf_counter()
+    for _ in range(3):
+        flush.fill_(1.0)
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"metric": "device copy for counter calibration", "bytes_read_per_copy": 4 * n,
+            "bytes_written_per_copy": 4 * n, "copies": 3,
+            "wall_s": round(time.perf_counter() - t0, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -415,7 +434,7 @@ def main():
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batch", "serve", "precision", "thermal256",
-                             "vpipes1024"])
+                             "vpipes1024", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -426,7 +445,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     fn = {"headline": run_headline, "batch": run_batch, "precision": run_precision,
-          "thermal256": run_thermal256, "vpipes1024": run_vpipes1024}
+          "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "copy": run_copy}
     if args.workload == "serve":
         from highmap_amd import serve
 
