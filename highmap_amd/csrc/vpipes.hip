@@ -262,16 +262,47 @@ __global__ void __launch_bounds__(256) k_vp_sum(const typename S::T *__restrict_
 //   B(r)  sediment after transport               -- computed at row k+1
 //   C(r)  new fluxes (flow pass)                 -- computed at row k+1
 //   D(k)  water pass + erosion of row k, stores  -- computed at row k
-// Left/right neighbours come from other lanes (ds_bpermute) with clamped
-// source lanes, so the grid's clamp-to-edge reads hold at the left and right
-// edges; above and below, loads use clamped rows, and the computed rows
-// B/C(-1) and B/C(ny) are replaced by those of row 0 and ny-1.
+// Left/right neighbours come from the adjacent lanes through DPP wave
+// shifts (a VALU operand modifier, no LDS round trip).  Lanes left of
+// column 0 and right of column nx-1 load the clamped column, so a staged
+// A value fetched across a grid edge already equals the clamp-to-edge read;
+// computed values (fluxes C, sediment B) differ in those lanes and are
+// replaced by the lane's own value at the edge columns.  The wave's end
+// lanes 0 and 63 receive 0; they are halo lanes whose results are unused.
+// Above and below, loads use clamped rows, and the computed rows B/C(-1)
+// and B/C(ny) are replaced by those of row 0 and ny-1.
 
 constexpr int STRIP_OUT = 60; // columns written per wavefront
 
-__device__ __forceinline__ float lane_get(float v, int src_x4)
+// lane i receives lane i-1's value (wave_shr:1)
+__device__ __forceinline__ float from_left(float v)
 {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(src_x4, __float_as_int(v)));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+// lane i receives lane i+1's value (wave_shl:1)
+__device__ __forceinline__ float from_right(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+// The neighbour's value, or the lane's own where `own` holds.  A DPP read
+// from a lane that EXEC disables yields 0, and left to itself the compiler
+// turns `own ? v : from_left(v)` into a DPP move under EXEC = !own, which
+// disables exactly the edge lane the next lane reads.  The empty asm pins
+// the exchange to all lanes before the select.
+__device__ __forceinline__ float left_or_own(float v, bool own)
+{
+  float n = from_left(v);
+  asm volatile("" : "+v"(n));
+  return own ? v : n;
+}
+
+__device__ __forceinline__ float right_or_own(float v, bool own)
+{
+  float n = from_right(v);
+  asm volatile("" : "+v"(n));
+  return own ? v : n;
 }
 
 struct RowA
@@ -316,12 +347,15 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
 }
 
 // XCD-aware work order: workgroup b runs on XCD b % 8; give each XCD a
-// contiguous range of work items so halo reuse stays in its L2.
+// contiguous range of work items so halo reuse stays in its L2.  The item
+// is the same in all lanes of a wave; readfirstlane tells the compiler so,
+// and everything derived from it (strip, rows, map) lives in SGPRs with
+// scalar branches instead of EXEC-masked ones.
 __device__ __forceinline__ int xcd_wave_item()
 {
   const int G = gridDim.x;
   const int b = (G & 7) ? blockIdx.x : (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
-  return b * 4 + (threadIdx.x >> 6);
+  return __builtin_amdgcn_readfirstlane(b * 4 + (threadIdx.x >> 6));
 }
 
 // One wavefront's work item of one iteration of one map: a strip of 60
@@ -361,8 +395,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   const int x0 = strip * STRIP_OUT - 2; // grid column of lane 0
   const int gx = x0 + lane;
   const int gi = clampi(gx, nx);
-  const int srcl = min(max(clampi(gi - 1, nx) - x0, 0), 63) << 2;
-  const int srcr = min(max(clampi(gi + 1, nx) - x0, 0), 63) << 2;
+  const bool ledge = gx == 0, redge = gx == nx - 1;
   const bool writer = lane >= 2 && lane < 62 && gx < nx;
 
   const int j0 = chunk * chunk_rows;
@@ -384,7 +417,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
       t.s10 = t.s01 = t.s11 = 0.f;
       return t;
     }
-    float frl = lane_get(a.fr, srcl), flr = lane_get(a.fl, srcr);
+    float frl = from_left(a.fr), flr = from_right(a.fl);
     float u, v;
     vp_velocity(a.dr, frl, am.ft, flr, ap.fb, a.fl, a.fr, a.ft, a.fb, u, v, k);
     return vp_transport_issue<S>(Sp, gi, gj, u, v, k);
@@ -398,8 +431,8 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   };
   // C(r): flow of row r from A(r-1), A(r), A(r+1); rounded to storage
   auto flow = [&](const RowA &am, const RowA &a, const RowA &ap, float c[4]) {
-    float zl = lane_get(a.z, srcl), dl = lane_get(a.d1, srcl);
-    float zr = lane_get(a.z, srcr), dr = lane_get(a.d1, srcr);
+    float zl = from_left(a.z), dl = from_left(a.d1);
+    float zr = from_right(a.z), dr = from_right(a.d1);
     float l = a.fl, r = a.fr, t = a.ft, bb = a.fb;
     vp_flow(a.z, a.d1, zl, dl, zr, dr, ap.z, ap.d1, am.z, am.d1, l, r, t, bb, k);
     c[0] = rnd<S>(l);
@@ -408,73 +441,79 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
     c[3] = rnd<S>(bb);
   };
 
+  // The window lives in rings of U slots indexed by compile-time constants:
+  // the row loop is unrolled U times by hand (the compiler will not unroll
+  // a loop holding DPP exchanges, which are convergent), so rows rotate
+  // through the slots by renaming instead of register moves.  Row r + q of
+  // an unrolled pass finds A(r+q-1) in slot q, A(r+q) in q+1, ..., and
+  // B/C(r+q-1), B/C(r+q), B/C(r+q+1) in slots q, q+1, q+2 (mod U).  With
+  // the load-ahead row, five A rows are live at once.
+  constexpr int U = PF ? 5 : 4;
+  RowA  R[U];
+  float Sr[U], Cr[U][4];
+
   // pre-roll: A(j0-2..j0+1), B/C(j0-1), B/C(j0)
-  RowA  am2 = load(j0 - 2), am1 = load(j0 - 1), a0 = load(j0), ap1 = load(j0 + 1);
-  float s_m1 = transport(am2, am1, a0, j0 - 1);
-  float c_m1[4];
-  flow(am2, am1, a0, c_m1);
-  float s_0 = transport(am1, a0, ap1, j0);
-  float c_0[4];
-  flow(am1, a0, ap1, c_0);
+  {
+    RowA am2 = load(j0 - 2);
+    R[0] = load(j0 - 1);
+    R[1] = load(j0);
+    R[2] = load(j0 + 1);
+    Sr[0] = transport(am2, R[0], R[1], j0 - 1);
+    flow(am2, R[0], R[1], Cr[0]);
+    Sr[1] = transport(R[0], R[1], R[2], j0);
+    flow(R[0], R[1], R[2], Cr[1]);
+    if (PF) R[3] = load(j0 + 2);
+  }
 
   Fix128 acc{0, 0};
-  RowA   apf;
-  if (PF) apf = load(j0 + 2);
-#pragma unroll 2
-  for (int r = j0; r < j1; r++)
+  for (int r0 = j0; r0 < j1; r0 += U)
   {
-    RowA ap2;
-    if (PF)
-    {
-      // rows are loaded one iteration before they are used
-      ap2 = apf;
-      apf = load(r + 3);
-    }
-    else
-      ap2 = load(r + 2);
-    TexelFetch tf = transport_issue(a0, ap1, ap2, r + 1);
-    float      c_p1[4];
-    flow(a0, ap1, ap2, c_p1);
-    float s_p1 = transport_finish(tf);
-
-    // row r: water pass and erosion
-    const bool top = r == 0, bot = r == ny - 1;
-    float      s_lo = top ? s_0 : s_m1, s_hi = bot ? s_0 : s_p1;
-    float      ftb = top ? c_0[2] : c_m1[2], fbt = bot ? c_0[3] : c_p1[3];
-    float      frl = lane_get(c_0[1], srcl), flr = lane_get(c_0[0], srcr);
-    float      d2 = rnd<S>(vp_depth(a0.d1, frl, ftb, flr, fbt, c_0[0], c_0[1], c_0[2], c_0[3], k));
-    float      u, v;
-    vp_velocity(d2, frl, ftb, flr, fbt, c_0[0], c_0[1], c_0[2], c_0[3], u, v, k);
-    float zl = lane_get(a0.z, srcl), zr = lane_get(a0.z, srcr);
-    float sl = lane_get(s_0, srcl), sr = lane_get(s_0, srcr);
-    float zn, sn;
-    vp_erosion<KEXP1>(a0.z, s_0, zr, zl, ap1.z, am1.z, sr, sl, s_hi, s_lo, d2, u, v, zn, sn, k);
-    if (writer)
-    {
-      const uint32_t c = (uint32_t)(r * nx + gx);
-      at_off(zo, c) = zn;
-      at_off(So, c) = S::st(sn);
-      at_off(Do, c) = S::st(d2);
-      Flux4<S> o;
-      o.l = S::st(c_0[0]);
-      o.r = S::st(c_0[1]);
-      o.t = S::st(c_0[2]);
-      o.b = S::st(c_0[3]);
-      at_off(Fo, c) = o;
-      if (k.maintain) fix_add(acc, fix_term(d2 * k.ce));
-    }
-
-    // shift the window
-    am1 = a0;
-    a0 = ap1;
-    ap1 = ap2;
-    s_m1 = s_0;
-    s_0 = s_p1;
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < U; q++)
     {
-      c_m1[q] = c_0[q];
-      c_0[q] = c_p1[q];
+      const int r = r0 + q;
+      if (r >= j1) break;
+      RowA &am1 = R[q % U], &a0 = R[(q + 1) % U], &ap1 = R[(q + 2) % U];
+      RowA &ap2 = R[(q + 3) % U];
+      // rows are loaded one row before they are used with the load-ahead
+      if (PF)
+        R[(q + 4) % U] = load(r + 3);
+      else
+        ap2 = load(r + 2);
+      float &s_m1 = Sr[q % U], &s_0 = Sr[(q + 1) % U], &s_p1 = Sr[(q + 2) % U];
+      float *c_m1 = Cr[q % U], *c_0 = Cr[(q + 1) % U], *c_p1 = Cr[(q + 2) % U];
+
+      TexelFetch tf = transport_issue(a0, ap1, ap2, r + 1);
+      flow(a0, ap1, ap2, c_p1);
+      s_p1 = transport_finish(tf);
+
+      // row r: water pass and erosion
+      const bool top = r == 0, bot = r == ny - 1;
+      float      s_lo = top ? s_0 : s_m1, s_hi = bot ? s_0 : s_p1;
+      float      ftb = top ? c_0[2] : c_m1[2], fbt = bot ? c_0[3] : c_p1[3];
+      float      frl = left_or_own(c_0[1], ledge);
+      float      flr = right_or_own(c_0[0], redge);
+      float      d2 = rnd<S>(vp_depth(a0.d1, frl, ftb, flr, fbt, c_0[0], c_0[1], c_0[2], c_0[3], k));
+      float      u, v;
+      vp_velocity(d2, frl, ftb, flr, fbt, c_0[0], c_0[1], c_0[2], c_0[3], u, v, k);
+      float zl = from_left(a0.z), zr = from_right(a0.z);
+      float sl = left_or_own(s_0, ledge), sr = right_or_own(s_0, redge);
+      float zn, sn;
+      vp_erosion<KEXP1>(a0.z, s_0, zr, zl, ap1.z, am1.z, sr, sl, s_hi, s_lo, d2, u, v, zn, sn, k);
+      if (writer)
+      {
+        const uint32_t c = (uint32_t)(r * nx + gx);
+        at_off(zo, c) = zn;
+        at_off(So, c) = S::st(sn);
+        at_off(Do, c) = S::st(d2);
+        Flux4<S> o;
+        o.l = S::st(c_0[0]);
+        o.r = S::st(c_0[1]);
+        o.t = S::st(c_0[2]);
+        o.b = S::st(c_0[3]);
+        at_off(Fo, c) = o;
+        if (k.maintain) fix_add(acc, fix_term(d2 * k.ce));
+      }
     }
   }
 
