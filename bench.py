@@ -52,8 +52,21 @@ def cpu_bench(workload, nx, ny, iters, reps=3, threads=None):
     return json.loads(out.strip().splitlines()[-1])
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n, iters=20, reps=3):
     r = cpu_bench("vpipes_thermal", n, n, iters, reps)
+    # one thread on a shorter sample of the same grid (no thermal step falls in it)
+    r1 = cpu_bench("vpipes_thermal", n, n, 4, 1, 1)
     return {
         "value": round(r["gcell_iter_s"], 5),
         "unit": "Gcell-iter/s",
@@ -63,6 +76,11 @@ def cpu_baseline(n, iters=20, reps=3):
                   "FP32 CPU restatement of the reference passes built -O3 -Ofast -ffast-math, "
                   "OpenMP %d threads, median of %d runs after 1 warm-up (%.2f s each)"
                   % (n, n, iters, r["threads"], reps, r["median_s"]),
+        "value_1_thread": round(r1["gcell_iter_s"], 5),
+        "sample_1_thread": "same grid, 4 vpipes iterations, 1 thread, 1 run after 1 warm-up "
+                           "(%.2f s)" % r1["median_s"],
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
     }
 
 
@@ -306,11 +324,35 @@ def run_precision(args):
                           "mass_vs_fp64_rel": float(((z.astype(np.float64) + sed).sum() - m64)
                                                     / m64)}
             rows.append(row)
+            print("precision %s: iteration %d done" % (pname, cp), file=sys.stderr, flush=True)
         for s in sess.values():
             s.close()
         out["results"][pname] = {"params": params or "hmap::gpu::hydraulic_vpipes defaults",
                                  "checkpoints": rows,
                                  "oracle_s": round(time.perf_counter() - t0, 1)}
+
+    # The chunked form the reference API can express, once for the record:
+    # hydraulic_vpipes(iterations=10) then thermal(1), repeated, with water,
+    # sediment and fluxes reset by every call.  FP32 drop-in calls (host
+    # arrays) against the FP64 oracle run the same way.
+    t0 = time.perf_counter()
+    z32 = z0.copy()
+    z64 = z0.astype(np.float64)
+    p64 = O.vp_params(**WORKLOAD)
+    rows = []
+    for c in range(1, total // 10 + 1):
+        hm.hydraulic_vpipes(z32, iterations=10, **WORKLOAD)
+        hm.thermal(z32, 1.0 / n, 1)
+        z64 = O.hydraulic_vpipes(z64, p64, None, 10, np.float64)[0]
+        z64 = O.thermal(z64, 1.0 / n, 1, np.float64)
+        if 10 * c in checkpoints:
+            print("precision chunked: iteration %d done" % (10 * c), file=sys.stderr, flush=True)
+            rng = float(z64.max() - z64.min())
+            rows.append({"iteration": 10 * c,
+                         "f32": {"max_abs_err_rel_range": float(np.abs(z32 - z64).max() / rng)}})
+    out["results"]["chunked_dropin"] = {
+        "params": WORKLOAD, "form": "%d x (hydraulic_vpipes(iterations=10), thermal(1))"
+        % (total // 10), "checkpoints": rows, "oracle_s": round(time.perf_counter() - t0, 1)}
     return out
 
 
@@ -354,6 +396,60 @@ def run_thermal256(args):
             "max_abs_err_rel_range_vs_fp64": float(np.abs(z - o64).max() / (o64.max() - o64.min())),
             "cpu_1_thread_gcell_iter_s": c1["gcell_iter_s"],
             "cpu_all_threads_gcell_iter_s": cn["gcell_iter_s"], "cpu_threads": cn["threads"]}
+
+
+def run_flow(args):
+    """hmap::gpu::flow_simulation on 4096^2 fBm (seed 2) with the parameters of
+    the reference's example (water_height 1e-2, uniform depth map, dt 0.5,
+    flux diffusion 0.01), 200 iterations: device-resident and drop-in (host
+    arrays), checked bit for bit against the FP32 oracle; roofline of the
+    fused flow kernel at 44 B per cell-iteration (z read; d and 4 fluxes read
+    and written)."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+
+    torch, hm, ctx = _gpu()
+    n, iters = args.n, 200
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    z = device_terrain(torch, hm, ctx, n, n, 2)
+    dm = torch.ones_like(z)
+    d = torch.empty_like(z)
+
+    def resident(it):
+        hm._check(hm.lib().hmg_flow_simulation_dev(ctx.handle, z.data_ptr(), n, n, 1e-2,
+                                                   dm.data_ptr(), it, 0.5, 1, 0.01, 0.0,
+                                                   d.data_ptr()))
+
+    resident(5)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    best = 1e9
+    for _ in range(3):
+        e0.record(stream)
+        resident(iters)
+        e1.record(stream)
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) / 1e3)
+    zh = z.cpu().numpy()
+    t0 = time.perf_counter()
+    got = hm.flow_simulation(zh, 1e-2, np.ones_like(zh), iters)
+    dropin = time.perf_counter() - t0
+    print("flow: GPU done, running the oracle", file=sys.stderr, flush=True)
+    want = O.flow_simulation(zh, 1e-2, np.ones_like(zh), iters)
+    c = cpu_bench("flow", n, n, 10, 3)
+    alg = 44 * n * n * iters
+    return {"metric": "Gcell-iter/s flow_simulation 4096^2 x 200", "unit": "Gcell-iter/s",
+            "n_gpus": 1, "value": round(n * n * iters / best / 1e9, 3),
+            "dropin_gcell_iter_s": round(n * n * iters / dropin / 1e9, 3),
+            "bitexact_vs_fp32_oracle": bool(np.array_equal(got, want)),
+            "roofline": {"kernel": "k_flow_stream", "bound": "hbm",
+                         "achieved": round(alg / best / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / best / 1e9 / HBM_PEAK_GBS, 4),
+                         "avg_launch_us": round(best / iters * 1e6, 2)},
+            "cpu_baseline": {"value": round(c["gcell_iter_s"], 5), "unit": "Gcell-iter/s",
+                             "cores": c["threads"], "kind": "port",
+                             "sample": "4096^2, 10 iterations, median of 3"}}
 
 
 def run_vpipes1024(args):
@@ -431,10 +527,12 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--iters", type=int, default=500)
     ap.add_argument("--maps", type=int, default=0)
-    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--max-active-cells", type=int, default=3 * 4096 * 4096,
+                    help="serve: cells of the jobs that run at once (0: no cap)")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batch", "serve", "precision", "thermal256",
-                             "vpipes1024", "copy"])
+                             "vpipes1024", "flow", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -445,7 +543,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     fn = {"headline": run_headline, "batch": run_batch, "precision": run_precision,
-          "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "copy": run_copy}
+          "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "flow": run_flow,
+          "copy": run_copy}
     if args.workload == "serve":
         from highmap_amd import serve
 

@@ -149,6 +149,13 @@ def lib():
             "hmg_batch_run": ([vp, i32, C.POINTER(f32)], i32),
             "hmg_batch_export": ([vp, i32, vp, vp, vp, vp, vp], i32),
             "hmg_batch_destroy": ([vp], i32),
+            "hmg_thermal_auto_bedrock": ([vp, vp, i32, i32, vp, f32, i32, vp, vp], i32),
+            "hmg_thermal_auto_bedrock_dev": ([vp, vp, i32, i32, vp, f32, i32], i32),
+            "hmg_sediment_deposition": ([vp, vp, i32, i32, vp, f32, vp, f32, i32, i32, vp], i32),
+            "hmg_flow_simulation": ([vp, vp, i32, i32, f32, vp, i32, f32, i32, f32, f32, vp],
+                                    i32),
+            "hmg_flow_simulation_dev": ([vp, vp, i32, i32, f32, vp, i32, f32, i32, f32, f32,
+                                         vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name, None)
@@ -276,6 +283,47 @@ def thermal(z, talus, iterations=10, bedrock=None, deposition_map=None, mask=Non
     return z
 
 
+def _talus(talus, shape):
+    if np.isscalar(talus):
+        return None, float(talus)
+    return _f32(talus, shape, "talus"), 0.0
+
+
+def thermal_auto_bedrock(z, talus, iterations=10, deposition_map=None, mask=None, ctx=None):
+    """hmap::gpu::thermal_auto_bedrock (erosion/thermal_gpu.cpp:98-146); z is
+    updated in place.  `talus` is a scalar or an array; `mask` gives the
+    masked overload."""
+    z = _inplace(z)
+    ny, nx = z.shape
+    t, ts = _talus(talus, (ny, nx))
+    m = _f32(mask, (ny, nx), "mask")
+    if deposition_map is not None:
+        deposition_map = _inplace(deposition_map)
+        if deposition_map.shape != z.shape:
+            raise ValueError("deposition_map shape mismatch")
+    _check(lib().hmg_thermal_auto_bedrock(_ctx(ctx).handle, _ptr(z), nx, ny, _ptr(t), ts,
+                                          int(iterations), _ptr(deposition_map), _ptr(m)))
+    return z
+
+
+def sediment_deposition(z, talus, deposition_map=None, max_deposition=0.01, iterations=5,
+                        thermal_subiterations=10, mask=None, ctx=None):
+    """hmap::gpu::sediment_deposition (erosion/deposition.cpp:19-63); z is
+    updated in place."""
+    z = _inplace(z)
+    ny, nx = z.shape
+    t, ts = _talus(talus, (ny, nx))
+    m = _f32(mask, (ny, nx), "mask")
+    if deposition_map is not None:
+        deposition_map = _inplace(deposition_map)
+        if deposition_map.shape != z.shape:
+            raise ValueError("deposition_map shape mismatch")
+    _check(lib().hmg_sediment_deposition(_ctx(ctx).handle, _ptr(z), nx, ny, _ptr(t), ts,
+                                         _ptr(deposition_map), float(max_deposition),
+                                         int(iterations), int(thermal_subiterations), _ptr(m)))
+    return z
+
+
 # ----------------------------------------------------------------- vpipes
 
 
@@ -310,6 +358,23 @@ def hydraulic_vpipes(z, water_height=1e-2, maintain_water_volume=True, evap_rate
     _check(lib().hmg_vpipes(_ctx(ctx).handle, _ptr(z), nx, ny, C.byref(p), _ptr(r),
                             *[_ptr(a) for a in outs]))
     return z
+
+
+def flow_simulation(z, water_height, depth_map, iterations, dt=0.5, flux_diffusion=True,
+                    flux_diffusion_strength=0.01, dry_out_ratio=0.0, ctx=None):
+    """hmap::gpu::flow_simulation (hydrology/flow_simulation.cpp:26-110):
+    returns the water depth after `iterations` flow + water passes starting
+    from water_height * depth_map, then the optional dry-out.  z and
+    depth_map are not changed."""
+    zz = _f32(z, None, "z")
+    ny, nx = zz.shape
+    dm = _f32(depth_map, (ny, nx), "depth_map")
+    d = np.empty((ny, nx), np.float32)
+    _check(lib().hmg_flow_simulation(_ctx(ctx).handle, _ptr(zz), nx, ny, float(water_height),
+                                     _ptr(dm), int(iterations), float(dt),
+                                     int(bool(flux_diffusion)), float(flux_diffusion_strength),
+                                     float(dry_out_ratio), _ptr(d)))
+    return d
 
 
 class Session:

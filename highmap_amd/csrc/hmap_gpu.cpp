@@ -189,6 +189,90 @@ void thermal(Array &z, float talus, int iterations, Array *p_bedrock, Array *p_d
         "thermal");
 }
 
+// HighMap/src/erosion/thermal_gpu.cpp:98-146
+void thermal_auto_bedrock(Array &z, const Array &talus, int iterations, Array *p_deposition_map)
+{
+  thermal_auto_bedrock(z, nullptr, talus, iterations, p_deposition_map);
+}
+
+void thermal_auto_bedrock(Array       &z,
+                          const Array *p_mask,
+                          const Array &talus,
+                          int          iterations,
+                          Array       *p_deposition_map)
+{
+  check_shape(talus, z.shape, "talus");
+  if (p_mask) check_shape(*p_mask, z.shape, "mask");
+  if (p_deposition_map) *p_deposition_map = Array(z.shape);
+  check(hmg_thermal_auto_bedrock(ctx(),
+                                 z.vector.data(),
+                                 z.shape.x,
+                                 z.shape.y,
+                                 talus.vector.data(),
+                                 0.f,
+                                 iterations,
+                                 p_deposition_map ? p_deposition_map->vector.data() : nullptr,
+                                 p_mask ? p_mask->vector.data() : nullptr),
+        "thermal_auto_bedrock");
+}
+
+void thermal_auto_bedrock(Array &z, float talus, int iterations, Array *p_deposition_map)
+{
+  if (p_deposition_map) *p_deposition_map = Array(z.shape);
+  check(hmg_thermal_auto_bedrock(ctx(),
+                                 z.vector.data(),
+                                 z.shape.x,
+                                 z.shape.y,
+                                 nullptr,
+                                 talus,
+                                 iterations,
+                                 p_deposition_map ? p_deposition_map->vector.data() : nullptr,
+                                 nullptr),
+        "thermal_auto_bedrock");
+}
+
+// HighMap/src/erosion/deposition.cpp:19-63
+void sediment_deposition(Array       &z,
+                         Array       *p_mask,
+                         const Array &talus,
+                         Array       *p_deposition_map,
+                         float        max_deposition,
+                         int          iterations,
+                         int          thermal_subiterations)
+{
+  check_shape(talus, z.shape, "talus");
+  if (p_mask) check_shape(*p_mask, z.shape, "mask");
+  if (p_deposition_map) *p_deposition_map = Array(z.shape);
+  check(hmg_sediment_deposition(ctx(),
+                                z.vector.data(),
+                                z.shape.x,
+                                z.shape.y,
+                                talus.vector.data(),
+                                0.f,
+                                p_deposition_map ? p_deposition_map->vector.data() : nullptr,
+                                max_deposition,
+                                iterations,
+                                thermal_subiterations,
+                                p_mask ? p_mask->vector.data() : nullptr),
+        "sediment_deposition");
+}
+
+void sediment_deposition(Array       &z,
+                         const Array &talus,
+                         Array       *p_deposition_map,
+                         float        max_deposition,
+                         int          iterations,
+                         int          thermal_subiterations)
+{
+  sediment_deposition(z,
+                      nullptr,
+                      talus,
+                      p_deposition_map,
+                      max_deposition,
+                      iterations,
+                      thermal_subiterations);
+}
+
 void hydraulic_vpipes(Array &z,
                       float  water_height,
                       bool   maintain_water_volume,
@@ -238,6 +322,34 @@ void hydraulic_vpipes(Array &z,
                    p_vel_u ? p_vel_u->vector.data() : nullptr,
                    p_vel_v ? p_vel_v->vector.data() : nullptr),
         "hydraulic_vpipes");
+}
+
+// HighMap/src/hydrology/flow_simulation.cpp:26-110
+Array flow_simulation(const Array &z,
+                      float        water_height,
+                      const Array &depth_map,
+                      int          iterations,
+                      float        dt,
+                      bool         flux_diffusion,
+                      float        flux_diffusion_strength,
+                      float        dry_out_ratio)
+{
+  check_shape(depth_map, z.shape, "depth_map");
+  Array d(z.shape);
+  check(hmg_flow_simulation(ctx(),
+                            z.vector.data(),
+                            z.shape.x,
+                            z.shape.y,
+                            water_height,
+                            depth_map.vector.data(),
+                            iterations,
+                            dt,
+                            flux_diffusion ? 1 : 0,
+                            flux_diffusion_strength,
+                            dry_out_ratio,
+                            d.vector.data()),
+        "flow_simulation");
+  return d;
 }
 
 } // namespace gpu

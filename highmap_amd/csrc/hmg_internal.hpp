@@ -133,6 +133,10 @@ int launch_thermal(hmg_ctx     *ctx,
                    hipStream_t  st);
 int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
 
+// Per-block min and max of a[0..n) into mm[2 + 2b], mm[3 + 2b] for the nblk
+// blocks of the launch (nblk <= 1024; mm holds 2 + 2 * nblk floats).
+int launch_minmax(const float *a, size_t n, float *mm, int nblk, hipStream_t st);
+
 } // namespace hmg
 
 // One erosion state resident on the device (see vpipes.hip).

@@ -237,6 +237,12 @@ __global__ void __launch_bounds__(256) k_remap(float *__restrict__ a,
     a[k] = flat ? vmin : a[k] * scale + offset;
 }
 
+int launch_minmax(const float *a, size_t n, float *mm, int nblk, hipStream_t st)
+{
+  k_minmax<<<nblk, 256, 0, st>>>(a, n, mm);
+  return check_launch("minmax");
+}
+
 } // namespace hmg
 
 using namespace hmg;

@@ -53,6 +53,7 @@ struct hmg_server
   int                           storage = HMG_STORAGE_F32;
   hmg_vpipes_params             p{};
   size_t                        budget = 0, used = 0;
+  size_t                        max_cells = 0, active_cells = 0; // 0: no cap
   std::mutex                    mu;
   std::condition_variable       cv, cv_done;
   std::deque<Job *>             queue;
@@ -175,11 +176,14 @@ void worker_main(hmg_server *sv)
       // admission in arrival order while the state fits in the budget
       while (!sv->queue.empty())
       {
-        Job *j = sv->queue.front();
-        if (sv->used + j->bytes > sv->budget && !(sv->active.empty() && admitted.empty()))
-          break;
+        Job         *j = sv->queue.front();
+        const bool   alone = sv->active.empty() && admitted.empty();
+        const size_t cells = (size_t)j->desc.nx * j->desc.ny;
+        if (sv->used + j->bytes > sv->budget && !alone) break;
+        if (sv->max_cells && sv->active_cells + cells > sv->max_cells && !alone) break;
         sv->queue.pop_front();
         sv->used += j->bytes;
+        sv->active_cells += cells;
         admitted.push_back(j);
       }
     }
@@ -231,6 +235,7 @@ void worker_main(hmg_server *sv)
       for (Job *j : finished)
       {
         sv->used -= j->bytes;
+        sv->active_cells -= (size_t)j->desc.nx * j->desc.ny;
         sv->active.erase(std::find(sv->active.begin(), sv->active.end(), j));
       }
       sv->cell_iters += work;
@@ -341,6 +346,17 @@ int hmg_server_job_times(hmg_server *sv,
   if (t_submit) *t_submit = it->second->t_submit;
   if (t_admit) *t_admit = it->second->t_admit;
   if (t_done) *t_done = it->second->t_done;
+  return HMG_OK;
+}
+
+int hmg_server_set_max_active_cells(hmg_server *sv, size_t cells)
+{
+  if (!sv) return set_error(HMG_ERR_INVALID, "null server");
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->max_cells = cells;
+  }
+  sv->cv.notify_all();
   return HMG_OK;
 }
 

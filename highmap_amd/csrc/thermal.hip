@@ -20,6 +20,9 @@
 //                    written once per T steps, so HBM traffic falls to
 //                    ~8/T B per cell-step plus the halo overhead.
 //  k_extrap_cols / k_extrap_rows  extrapolate_borders, columns then rows.
+#include <algorithm>
+#include <cfloat>
+
 #include "hmg_internal.hpp"
 
 namespace hmg
@@ -80,6 +83,116 @@ __global__ void __launch_bounds__(256) k_thermal_step(const float *__restrict__ 
     }
   }
   zn[idx] = out;
+}
+
+// thermal_auto_bedrock (thermal.cl:102-145), one Jacobi step.  bed is the
+// per-cell bedrock (read and written by its own cell only), z0 the input of
+// the call; `first` marks step 0, where the kernel sets bedrock = -FLT_MAX.
+__global__ void __launch_bounds__(256) k_thermal_auto_step(const float *__restrict__ z,
+                                                           float *__restrict__ zn,
+                                                           int nx,
+                                                           int ny,
+                                                           const float *__restrict__ talus,
+                                                           float ts,
+                                                           float *__restrict__ bed,
+                                                           const float *__restrict__ z0,
+                                                           int first)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  size_t idx = (size_t)j * nx + i;
+  float  out;
+  if (i == 0)
+    out = z[idx + 1];
+  else if (i == nx - 1)
+    out = z[idx - 1];
+  else if (j == 0)
+    out = z[idx + nx];
+  else if (j == ny - 1)
+    out = z[idx - nx];
+  else
+  {
+    const float zb = first ? -FLT_MAX : bed[idx];
+    const float val = z[idx];
+    out = val;
+    if (val >= zb)
+    {
+      float n[8];
+      th_gather([&](int di, int dj) { return z[(size_t)((long)idx + (long)dj * nx + di)]; }, n);
+      const float z0v = z0[idx];
+      out = fmaxf(fmaxf(th_interior(val, n, talus ? talus[idx] : ts), z0v), zb);
+      bed[idx] = fmaxf(z0v, out);
+    }
+    else if (first)
+      bed[idx] = zb;
+  }
+  zn[idx] = out;
+}
+
+// After a thermal call: deposition = max(z - z0, 0) with std::max's rule
+// (thermal_gpu.cpp:71, range.cpp:253-262) if dep is given, then
+// apply_with_mask's lerp z = z0 * (1 - t) + z * t (operator.hpp:405-418,
+// math.cpp:183-195) if mask is given.
+__global__ void __launch_bounds__(256) k_thermal_post(float *__restrict__ z,
+                                                      const float *__restrict__ z0,
+                                                      float *__restrict__ dep,
+                                                      const float *__restrict__ mask,
+                                                      size_t n)
+{
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+  {
+    const float zk = z[k], z0k = z0[k];
+    if (dep)
+    {
+      const float dz = zk - z0k;
+      dep[k] = dz < 0.f ? 0.f : dz;
+    }
+    if (mask)
+    {
+      const float t = mask[k];
+      z[k] = z0k * (1.f - t) + zk * t;
+    }
+  }
+}
+
+// sediment_deposition (deposition.cpp:19-38) element-wise parts.
+__global__ void __launch_bounds__(256) k_sd_pre(float *__restrict__ smap,
+                                                const float *__restrict__ z,
+                                                float *__restrict__ ztot,
+                                                float  step,
+                                                size_t n)
+{
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+  {
+    const float sk = smap[k] + step; // smap = smap + deposition_step
+    smap[k] = sk;
+    ztot[k] = z[k] + sk; // z_tot = z + smap
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sd_post(float *__restrict__ smap,
+                                                 const float *__restrict__ ztot,
+                                                 const float *__restrict__ z,
+                                                 size_t n)
+{
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+  {
+    const float dz = ztot[k] - z[k]; // smap = maximum(z_tot - z, 0.f)
+    smap[k] = dz < 0.f ? 0.f : dz;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sd_finish(float *__restrict__ z,
+                                                   const float *__restrict__ smap,
+                                                   size_t n)
+{
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    z[k] = z[k] + smap[k]; // z = z + smap
 }
 
 // Temporal blocking.  A block owns a TW x TH output tile and loads the tile
@@ -226,6 +339,43 @@ int launch_thermal(hmg_ctx     *ctx,
   return HMG_OK;
 }
 
+// thermal_auto_bedrock steps: z holds the input (it is also z0, copied to
+// z0buf first) and receives the result; zscratch and bed are planes of the
+// same size.
+int launch_thermal_auto(float       *z,
+                        float       *zscratch,
+                        float       *bed,
+                        float       *z0buf,
+                        int          nx,
+                        int          ny,
+                        const float *talus,
+                        float        talus_scalar,
+                        int          iterations,
+                        hipStream_t  st)
+{
+  const size_t bytes = (size_t)nx * ny * sizeof(float);
+  HMG_HIP(hipMemcpyAsync(z0buf, z, bytes, hipMemcpyDeviceToDevice, st));
+  float *a = z, *b = zscratch;
+  dim3   blk(64, 4);
+  dim3   grd((nx + 63) / 64, (ny + 3) / 4);
+  for (int it = 0; it < iterations; it++)
+  {
+    k_thermal_auto_step<<<grd, blk, 0, st>>>(a, b, nx, ny, talus, talus_scalar, bed, z0buf,
+                                             it == 0 ? 1 : 0);
+    HMG_TRY(check_launch("thermal_auto_step"));
+    float *t = a;
+    a = b;
+    b = t;
+  }
+  if (a != z) HMG_HIP(hipMemcpyAsync(z, a, bytes, hipMemcpyDeviceToDevice, st));
+  return HMG_OK;
+}
+
+static int post_blocks(size_t n)
+{
+  return (int)std::min<size_t>((n + 255) / 256, 4096);
+}
+
 static int check_thermal_args(int nx, int ny, int iterations)
 {
   if (nx < 3 || ny < 3) return set_error(HMG_ERR_INVALID, "thermal needs nx, ny >= 3");
@@ -272,6 +422,81 @@ int hmg_thermal_dev(hmg_ctx     *ctx,
   return HMG_OK;
 }
 
+// Host-pointer form shared by thermal and thermal_auto_bedrock: upload,
+// steps, extrapolate_borders, then deposition and mask on the device.
+static int thermal_host_call(hmg_ctx     *ctx,
+                             bool         auto_bedrock,
+                             float       *z_host,
+                             int          nx,
+                             int          ny,
+                             const float *talus_host,
+                             float        talus_scalar,
+                             int          iterations,
+                             const float *bedrock_host,
+                             float       *deposition_host,
+                             const float *mask_host)
+{
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  hipStream_t  st = ctx->stream;
+  const size_t n = (size_t)nx * ny;
+  const size_t bytes = n * sizeof(float);
+  DevBuf       zb, z0, tmp, tb, bb, db, mb;
+  HMG_TRY(zb.get(ctx->pool, bytes));
+  HMG_TRY(z0.get(ctx->pool, bytes));
+  HMG_TRY(tmp.get(ctx->pool, bytes));
+  const float *pt = nullptr, *pb = nullptr, *pm = nullptr;
+  float       *pd = nullptr;
+  HMG_HIP(hipMemcpyAsync(zb.ptr, z_host, bytes, hipMemcpyHostToDevice, st));
+  if (talus_host)
+  {
+    HMG_TRY(tb.get(ctx->pool, bytes));
+    HMG_HIP(hipMemcpyAsync(tb.ptr, talus_host, bytes, hipMemcpyHostToDevice, st));
+    pt = tb.as<float>();
+  }
+  if (bedrock_host)
+  {
+    HMG_TRY(bb.get(ctx->pool, bytes));
+    HMG_HIP(hipMemcpyAsync(bb.ptr, bedrock_host, bytes, hipMemcpyHostToDevice, st));
+    pb = bb.as<float>();
+  }
+  if (mask_host)
+  {
+    HMG_TRY(mb.get(ctx->pool, bytes));
+    HMG_HIP(hipMemcpyAsync(mb.ptr, mask_host, bytes, hipMemcpyHostToDevice, st));
+    pm = mb.as<float>();
+  }
+  if (deposition_host)
+  {
+    HMG_TRY(db.get(ctx->pool, bytes));
+    pd = db.as<float>();
+  }
+  if (auto_bedrock)
+  {
+    DevBuf bed;
+    HMG_TRY(bed.get(ctx->pool, bytes));
+    HMG_TRY(launch_thermal_auto(zb.as<float>(), tmp.as<float>(), bed.as<float>(), z0.as<float>(),
+                                nx, ny, pt, talus_scalar, iterations, st));
+    HMG_HIP(hipStreamSynchronize(st)); // bed returns to the pool here
+  }
+  else
+  {
+    HMG_HIP(hipMemcpyAsync(z0.ptr, zb.ptr, bytes, hipMemcpyDeviceToDevice, st));
+    HMG_TRY(launch_thermal(ctx, zb.as<float>(), tmp.as<float>(), nx, ny, pt, talus_scalar,
+                           iterations, pb, st));
+  }
+  HMG_TRY(launch_extrapolate_borders(zb.as<float>(), nx, ny, st));
+  if (pd || pm)
+  {
+    k_thermal_post<<<post_blocks(n), 256, 0, st>>>(zb.as<float>(), z0.as<float>(), pd, pm, n);
+    HMG_TRY(check_launch("thermal_post"));
+  }
+  HMG_HIP(hipMemcpyAsync(z_host, zb.ptr, bytes, hipMemcpyDeviceToHost, st));
+  if (pd) HMG_HIP(hipMemcpyAsync(deposition_host, pd, bytes, hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  return HMG_OK;
+}
+
 int hmg_thermal(hmg_ctx     *ctx,
                 float       *z_host,
                 int          nx,
@@ -285,53 +510,125 @@ int hmg_thermal(hmg_ctx     *ctx,
 {
   clear_error();
   HMG_TRY(check_thermal_args(nx, ny, iterations));
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!z_host) return set_error(HMG_ERR_INVALID, "null z");
+  return thermal_host_call(ctx, false, z_host, nx, ny, talus_host, talus_scalar, iterations,
+                           bedrock_host, deposition_host, mask_host);
+}
+
+int hmg_thermal_auto_bedrock(hmg_ctx     *ctx,
+                             float       *z_host,
+                             int          nx,
+                             int          ny,
+                             const float *talus_host,
+                             float        talus_scalar,
+                             int          iterations,
+                             float       *deposition_host,
+                             const float *mask_host)
+{
+  clear_error();
+  HMG_TRY(check_thermal_args(nx, ny, iterations));
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!z_host) return set_error(HMG_ERR_INVALID, "null z");
+  return thermal_host_call(ctx, true, z_host, nx, ny, talus_host, talus_scalar, iterations,
+                           nullptr, deposition_host, mask_host);
+}
+
+int hmg_thermal_auto_bedrock_dev(hmg_ctx     *ctx,
+                                 float       *z_dev,
+                                 int          nx,
+                                 int          ny,
+                                 const float *talus_dev,
+                                 float        talus_scalar,
+                                 int          iterations)
+{
+  clear_error();
+  HMG_TRY(check_thermal_args(nx, ny, iterations));
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!z_dev) return set_error(HMG_ERR_INVALID, "null z");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  const size_t bytes = (size_t)nx * ny * sizeof(float);
+  DevBuf       tmp, bed, z0;
+  HMG_TRY(tmp.get(ctx->pool, bytes));
+  HMG_TRY(bed.get(ctx->pool, bytes));
+  HMG_TRY(z0.get(ctx->pool, bytes));
+  HMG_TRY(launch_thermal_auto(z_dev, tmp.as<float>(), bed.as<float>(), z0.as<float>(), nx, ny,
+                              talus_dev, talus_scalar, iterations, ctx->stream));
+  HMG_TRY(launch_extrapolate_borders(z_dev, nx, ny, ctx->stream));
+  HMG_HIP(hipStreamSynchronize(ctx->stream));
+  return HMG_OK;
+}
+
+int hmg_sediment_deposition(hmg_ctx     *ctx,
+                            float       *z_host,
+                            int          nx,
+                            int          ny,
+                            const float *talus_host,
+                            float        talus_scalar,
+                            float       *deposition_host,
+                            float        max_deposition,
+                            int          iterations,
+                            int          thermal_subiterations,
+                            const float *mask_host)
+{
+  clear_error();
+  HMG_TRY(check_thermal_args(nx, ny, thermal_subiterations));
+  if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
   if (!z_host) return set_error(HMG_ERR_INVALID, "null z");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   HMG_TRY(ctx_enter(ctx));
+  hipStream_t  st = ctx->stream;
   const size_t n = (size_t)nx * ny;
   const size_t bytes = n * sizeof(float);
-  DevBuf       zb, tmp, tb, bb;
+  DevBuf       zb, z0, smap, ztot, tmp, tb, mb;
   HMG_TRY(zb.get(ctx->pool, bytes));
+  HMG_TRY(smap.get(ctx->pool, bytes));
+  HMG_TRY(ztot.get(ctx->pool, bytes));
   HMG_TRY(tmp.get(ctx->pool, bytes));
-  const float *pt = nullptr, *pb = nullptr;
-  HMG_HIP(hipMemcpyAsync(zb.ptr, z_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  const float *pt = nullptr, *pm = nullptr;
+  HMG_HIP(hipMemcpyAsync(zb.ptr, z_host, bytes, hipMemcpyHostToDevice, st));
   if (talus_host)
   {
     HMG_TRY(tb.get(ctx->pool, bytes));
-    HMG_HIP(hipMemcpyAsync(tb.ptr, talus_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HMG_HIP(hipMemcpyAsync(tb.ptr, talus_host, bytes, hipMemcpyHostToDevice, st));
     pt = tb.as<float>();
   }
-  if (bedrock_host)
-  {
-    HMG_TRY(bb.get(ctx->pool, bytes));
-    HMG_HIP(hipMemcpyAsync(bb.ptr, bedrock_host, bytes, hipMemcpyHostToDevice, ctx->stream));
-    pb = bb.as<float>();
-  }
-  HMG_TRY(launch_thermal(ctx, zb.as<float>(), tmp.as<float>(), nx, ny, pt, talus_scalar,
-                         iterations, pb, ctx->stream));
-  HMG_TRY(launch_extrapolate_borders(zb.as<float>(), nx, ny, ctx->stream));
-
-  std::vector<float> z0;
-  if (deposition_host || mask_host) z0.assign(z_host, z_host + n);
-  HMG_HIP(hipMemcpyAsync(z_host, zb.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  HMG_HIP(hipStreamSynchronize(ctx->stream));
-
-  // thermal_gpu.cpp:71: deposition = maximum(z - z_bckp, 0.f), i.e.
-  // std::max(dz, 0.f) per cell (range.cpp:253-262)
-  if (deposition_host)
-    for (size_t k = 0; k < n; k++)
-    {
-      float dz = z_host[k] - z0[k];
-      deposition_host[k] = (dz < 0.f) ? 0.f : dz;
-    }
-  // apply_with_mask (operator.hpp:405-418) with lerp (math.cpp:183-195):
-  // z = z0 * (1 - t) + z * t
   if (mask_host)
-    for (size_t k = 0; k < n; k++)
-    {
-      float t = mask_host[k];
-      z_host[k] = z0[k] * (1.f - t) + z_host[k] * t;
-    }
+  {
+    HMG_TRY(mb.get(ctx->pool, bytes));
+    HMG_TRY(z0.get(ctx->pool, bytes));
+    HMG_HIP(hipMemcpyAsync(mb.ptr, mask_host, bytes, hipMemcpyHostToDevice, st));
+    HMG_HIP(hipMemcpyAsync(z0.ptr, zb.ptr, bytes, hipMemcpyDeviceToDevice, st));
+    pm = mb.as<float>();
+  }
+  HMG_HIP(hipMemsetAsync(smap.ptr, 0, bytes, st));
+  // deposition_step = max_deposition / (int)iterations (float division)
+  const float step = max_deposition / (float)iterations;
+  const int   nb = post_blocks(n);
+  for (int it = 0; it < iterations; it++)
+  {
+    k_sd_pre<<<nb, 256, 0, st>>>(smap.as<float>(), zb.as<float>(), ztot.as<float>(), step, n);
+    HMG_TRY(check_launch("sediment_deposition(pre)"));
+    HMG_TRY(launch_thermal(ctx, ztot.as<float>(), tmp.as<float>(), nx, ny, pt, talus_scalar,
+                           thermal_subiterations, nullptr, st));
+    HMG_TRY(launch_extrapolate_borders(ztot.as<float>(), nx, ny, st));
+    k_sd_post<<<nb, 256, 0, st>>>(smap.as<float>(), ztot.as<float>(), zb.as<float>(), n);
+    HMG_TRY(check_launch("sediment_deposition(post)"));
+  }
+  k_sd_finish<<<nb, 256, 0, st>>>(zb.as<float>(), smap.as<float>(), n);
+  HMG_TRY(check_launch("sediment_deposition(finish)"));
+  if (pm)
+  {
+    // the masked overload: z = lerp(z, z_f, mask) (deposition.cpp:47-62)
+    k_thermal_post<<<nb, 256, 0, st>>>(zb.as<float>(), z0.as<float>(), nullptr, pm, n);
+    HMG_TRY(check_launch("sediment_deposition(mask)"));
+  }
+  HMG_HIP(hipMemcpyAsync(z_host, zb.ptr, bytes, hipMemcpyDeviceToHost, st));
+  if (deposition_host)
+    HMG_HIP(hipMemcpyAsync(deposition_host, smap.ptr, bytes, hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
   return HMG_OK;
 }
 

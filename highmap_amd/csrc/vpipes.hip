@@ -310,7 +310,7 @@ struct RowA
   float z, d1, dr, fl, fr, ft, fb;
 };
 
-template <class S, bool URAIN>
+template <class S, bool URAIN, bool FLOW = false>
 __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
                                             const typename S::T *__restrict__ D,
                                             const Flux4<S> *__restrict__ F,
@@ -329,7 +329,9 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
   a.fr = S::ld(f.r);
   a.ft = S::ld(f.t);
   a.fb = S::ld(f.b);
-  if (URAIN)
+  if (FLOW)
+    a.d1 = a.dr; // flow_simulation: no evaporation, top-up or rain
+  else if (URAIN)
   {
     // vp_d1 with rain == 1: (wta * 1) / R and rr * 1 are uniform
     float d = a.dr;
@@ -361,7 +363,10 @@ __device__ __forceinline__ int xcd_wave_item()
 // One wavefront's work item of one iteration of one map: a strip of 60
 // output columns over `chunk_rows` rows.  `item` numbers the map's items;
 // the wavefront of item 0 clears the slot set of the next iteration.
-template <class S, bool URAIN, bool PF, bool KEXP1>
+// FLOW: the flow and water passes only (flow_simulation): d1 is the stored
+// depth, no transport, erosion or volume sum; only D and F are written
+// (Sp, rain, zo, So and the slot sets are not touched).
+template <class S, bool URAIN, bool PF, bool KEXP1, bool FLOW = false>
 __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
                                                const typename S::T *__restrict__ D,
                                                const Flux4<S> *__restrict__ F,
@@ -382,11 +387,11 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
 {
   const int nx = k.nx, ny = k.ny;
   const int lane = threadIdx.x & 63;
-  if (k.maintain && item == 0 && lane < NSLOT) clear_set[lane] = Fix128{0, 0};
+  if (!FLOW && k.maintain && item == 0 && lane < NSLOT) clear_set[lane] = Fix128{0, 0};
   const int strip = item % strips, chunk = item / strips;
 
   float wta = 0.f, topup_add = 0.f;
-  if (y.has_prev && y.topup)
+  if (!FLOW && y.has_prev && y.topup)
   {
     wta = slots_wta(prev_set, y.W0);
     topup_add = (wta * 1.f) / y.rf;
@@ -402,8 +407,8 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   const int j1 = min(j0 + chunk_rows, ny);
 
   auto load = [&](int r) {
-    return stream_load<S, URAIN>(z, D, F, rain, (uint32_t)(clampi(r, ny) * nx + gi), topup_add,
-                                 wta, k, y);
+    return stream_load<S, URAIN, FLOW>(z, D, F, rain, (uint32_t)(clampi(r, ny) * nx + gi),
+                                       topup_add, wta, k, y);
   };
   // B(r): transport of row r from A(r-1), A(r), A(r+1), in two halves
   auto transport_issue = [&](const RowA &am, const RowA &a, const RowA &ap, int r) {
@@ -458,9 +463,9 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
     R[0] = load(j0 - 1);
     R[1] = load(j0);
     R[2] = load(j0 + 1);
-    Sr[0] = transport(am2, R[0], R[1], j0 - 1);
+    if (!FLOW) Sr[0] = transport(am2, R[0], R[1], j0 - 1);
     flow(am2, R[0], R[1], Cr[0]);
-    Sr[1] = transport(R[0], R[1], R[2], j0);
+    if (!FLOW) Sr[1] = transport(R[0], R[1], R[2], j0);
     flow(R[0], R[1], R[2], Cr[1]);
     if (PF) R[3] = load(j0 + 2);
   }
@@ -483,6 +488,28 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
       float &s_m1 = Sr[q % U], &s_0 = Sr[(q + 1) % U], &s_p1 = Sr[(q + 2) % U];
       float *c_m1 = Cr[q % U], *c_0 = Cr[(q + 1) % U], *c_p1 = Cr[(q + 2) % U];
 
+      if (FLOW)
+      {
+        // flow_simulation: the water pass of row r, then D and F
+        flow(a0, ap1, ap2, c_p1);
+        const bool top = r == 0, bot = r == ny - 1;
+        float      ftb = top ? c_0[2] : c_m1[2], fbt = bot ? c_0[3] : c_p1[3];
+        float      frl = left_or_own(c_0[1], ledge);
+        float      flr = right_or_own(c_0[0], redge);
+        float d2 = rnd<S>(vp_depth(a0.d1, frl, ftb, flr, fbt, c_0[0], c_0[1], c_0[2], c_0[3], k));
+        if (writer)
+        {
+          const uint32_t c = (uint32_t)(r * nx + gx);
+          at_off(Do, c) = S::st(d2);
+          Flux4<S> o;
+          o.l = S::st(c_0[0]);
+          o.r = S::st(c_0[1]);
+          o.t = S::st(c_0[2]);
+          o.b = S::st(c_0[3]);
+          at_off(Fo, c) = o;
+        }
+        continue;
+      }
       TexelFetch tf = transport_issue(a0, ap1, ap2, r + 1);
       flow(a0, ap1, ap2, c_p1);
       s_p1 = transport_finish(tf);
@@ -517,7 +544,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
     }
   }
 
-  if (k.maintain)
+  if (!FLOW && k.maintain)
   {
     for (int o = 32; o > 0; o >>= 1)
     {
@@ -1533,6 +1560,198 @@ int hmg_batch_destroy(hmg_batch *b)
   if (!b) return HMG_OK;
   for (hmg_session *s : b->maps) hmg_session_destroy(s);
   delete b;
+  return HMG_OK;
+}
+
+} // extern "C"
+
+// -------------------------------------------------------- flow_simulation
+//
+// hmap::gpu::flow_simulation (HighMap/src/hydrology/flow_simulation.cpp:26-110):
+// d = water_height * depth_map, fluxes 0; per iteration the flow pass and the
+// water pass (d <- d2; the velocities are not kept); then, when
+// dry_out_ratio != 0, water_depth_dry_out (water_depth.cpp:40-64) with
+// depth_max = max(d).  One fused launch per iteration (the streaming item
+// without transport and erosion): z is read, d and the fluxes ping-pong.
+
+namespace hmg
+{
+
+template <bool PF>
+__global__ void __launch_bounds__(256) k_flow_stream(const float *__restrict__ z,
+                                                     const float *__restrict__ D,
+                                                     const Flux4<StF32> *__restrict__ F,
+                                                     float *__restrict__ Do,
+                                                     Flux4<StF32> *__restrict__ Fo,
+                                                     int     strips,
+                                                     int     chunk_rows,
+                                                     int     n_items,
+                                                     VpConst k)
+{
+  const int item = xcd_wave_item();
+  if (item >= n_items) return;
+  const VpDyn y{0, 0, 0.0, 1.f}; // not read by the flow-only item
+  vp_stream_item<StF32, true, PF, true, true>(z, D, F, nullptr, nullptr, nullptr, Do, Fo, nullptr,
+                                              nullptr, nullptr, nullptr, item, strips,
+                                              chunk_rows, k, y);
+}
+
+__global__ void __launch_bounds__(256) k_flow_init(const float *__restrict__ depth_map,
+                                                   float                wh,
+                                                   float *__restrict__ d,
+                                                   Flux4<StF32> *__restrict__ F,
+                                                   size_t n)
+{
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride)
+  {
+    d[c] = wh * depth_map[c]; // Array d = water_height * depth_map
+    F[c] = Flux4<StF32>{0.f, 0.f, 0.f, 0.f};
+  }
+}
+
+// water_depth_dry_out without a mask: d = max(0, d - ratio * depth_max),
+// depth_max from the per-block maxima of launch_minmax.
+__global__ void __launch_bounds__(256) k_dry_out(const float *__restrict__ d,
+                                                 float *__restrict__ out,
+                                                 size_t n,
+                                                 const float *__restrict__ mm,
+                                                 int   nblk,
+                                                 float ratio)
+{
+  __shared__ float s_sub;
+  if (threadIdx.x == 0)
+  {
+    float hi = mm[3];
+    for (int b = 1; b < nblk; b++) hi = fmaxf(hi, mm[3 + 2 * b]);
+    s_sub = ratio * hi;
+  }
+  __syncthreads();
+  const float sub = s_sub;
+  size_t      stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride)
+    out[c] = fmaxf(0.f, d[c] - sub);
+}
+
+} // namespace hmg
+
+extern "C" {
+
+int hmg_flow_simulation_dev(hmg_ctx     *ctx,
+                            const float *z_dev,
+                            int          nx,
+                            int          ny,
+                            float        water_height,
+                            const float *depth_map_dev,
+                            int          iterations,
+                            float        dt,
+                            int          flux_diffusion,
+                            float        flux_diffusion_strength,
+                            float        dry_out_ratio,
+                            float       *d_out_dev)
+{
+  clear_error();
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!z_dev || !depth_map_dev || !d_out_dev) return set_error(HMG_ERR_INVALID, "null array");
+  if (nx < 1 || ny < 1) return set_error(HMG_ERR_INVALID, "shape must be positive");
+  if ((size_t)nx * (size_t)ny > ((size_t)1 << 31))
+    return set_error(HMG_ERR_INVALID, "grid larger than 2^31 cells");
+  if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
+  if (!(dt > 0.f)) return set_error(HMG_ERR_INVALID, "dt must be > 0");
+  hmg_vpipes_params prm;
+  hmg_vpipes_defaults(&prm);
+  prm.water_height = water_height;
+  prm.dt = dt;
+  prm.flux_diffusion = flux_diffusion ? 1 : 0;
+  prm.flux_diffusion_strength = flux_diffusion_strength;
+  prm.maintain_water_volume = 0;
+  prm.evap_rate = 0.f;
+  const VpConst k = make_vp_const(prm, nx, ny);
+
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  hipStream_t  st = ctx->stream;
+  const size_t n = (size_t)nx * ny;
+  DevBuf       D[2], F[2];
+  for (int h = 0; h < 2; h++)
+  {
+    HMG_TRY(D[h].get(ctx->pool, n * sizeof(float)));
+    HMG_TRY(F[h].get(ctx->pool, n * sizeof(Flux4<StF32>)));
+  }
+  const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  k_flow_init<<<nb, 256, 0, st>>>(depth_map_dev, water_height, D[0].as<float>(),
+                                  (Flux4<StF32> *)F[0].ptr, n);
+  HMG_TRY(check_launch("flow_init"));
+
+  // launch shape as for the sessions: about 16 wavefronts per CU, 8 to 64
+  // rows per wavefront
+  const int  strips = (nx + STRIP_OUT - 1) / STRIP_OUT;
+  const long target = 16L * (ctx->num_cus > 0 ? ctx->num_cus : 256);
+  long       ch = ((long)ny * strips + target - 1) / target;
+  ch = std::max(8L, std::min(64L, ch));
+  const int n_items = strips * (int)((ny + ch - 1) / ch);
+  const int G = (n_items + 3) / 4;
+  int       a = 0;
+  for (int it = 0; it < iterations; it++, a ^= 1)
+  {
+    k_flow_stream<true><<<G, 256, 0, st>>>(z_dev, D[a].as<float>(),
+                                           (const Flux4<StF32> *)F[a].ptr, D[a ^ 1].as<float>(),
+                                           (Flux4<StF32> *)F[a ^ 1].ptr, strips, (int)ch,
+                                           n_items, k);
+    HMG_TRY(check_launch("flow_stream"));
+  }
+  if (dry_out_ratio != 0.f)
+  {
+    int nblk = (int)std::min<size_t>((n + 255) / 256, 1024);
+    DevBuf mm;
+    HMG_TRY(mm.get(ctx->pool, sizeof(float) * (2 + 2 * (size_t)nblk)));
+    HMG_TRY(launch_minmax(D[a].as<float>(), n, mm.as<float>(), nblk, st));
+    k_dry_out<<<nblk, 256, 0, st>>>(D[a].as<float>(), d_out_dev, n, mm.as<float>(), nblk,
+                                    dry_out_ratio);
+    HMG_TRY(check_launch("dry_out"));
+    // the pooled buffers are reused by later calls: finish before they return
+    HMG_HIP(hipStreamSynchronize(st));
+    return HMG_OK;
+  }
+  HMG_HIP(hipMemcpyAsync(d_out_dev, D[a].ptr, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  return HMG_OK;
+}
+
+int hmg_flow_simulation(hmg_ctx     *ctx,
+                        const float *z_host,
+                        int          nx,
+                        int          ny,
+                        float        water_height,
+                        const float *depth_map_host,
+                        int          iterations,
+                        float        dt,
+                        int          flux_diffusion,
+                        float        flux_diffusion_strength,
+                        float        dry_out_ratio,
+                        float       *d_out_host)
+{
+  clear_error();
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!z_host || !depth_map_host || !d_out_host) return set_error(HMG_ERR_INVALID, "null array");
+  if (nx < 1 || ny < 1) return set_error(HMG_ERR_INVALID, "shape must be positive");
+  if ((size_t)nx * (size_t)ny > ((size_t)1 << 31))
+    return set_error(HMG_ERR_INVALID, "grid larger than 2^31 cells");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  hipStream_t  st = ctx->stream;
+  const size_t bytes = (size_t)nx * ny * sizeof(float);
+  DevBuf       z, dm, d;
+  HMG_TRY(z.get(ctx->pool, bytes));
+  HMG_TRY(dm.get(ctx->pool, bytes));
+  HMG_TRY(d.get(ctx->pool, bytes));
+  HMG_HIP(hipMemcpyAsync(z.ptr, z_host, bytes, hipMemcpyHostToDevice, st));
+  HMG_HIP(hipMemcpyAsync(dm.ptr, depth_map_host, bytes, hipMemcpyHostToDevice, st));
+  HMG_TRY(hmg_flow_simulation_dev(ctx, z.as<float>(), nx, ny, water_height, dm.as<float>(),
+                                  iterations, dt, flux_diffusion, flux_diffusion_strength,
+                                  dry_out_ratio, d.as<float>()));
+  HMG_HIP(hipMemcpyAsync(d_out_host, d.ptr, bytes, hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
   return HMG_OK;
 }
 

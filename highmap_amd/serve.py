@@ -40,19 +40,21 @@ def _bind():
     L.hmg_server_poll.argtypes = [vp, i64, C.POINTER(i32)]
     L.hmg_server_job_times.argtypes = [vp, i64, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
     L.hmg_server_wait_all.argtypes = [vp]
+    L.hmg_server_set_max_active_cells.argtypes = [vp, sz]
     L.hmg_server_stats.argtypes = [vp, C.POINTER(dbl), C.POINTER(i64), C.POINTER(dbl),
                                    C.POINTER(sz), C.POINTER(sz)]
     L.hmg_server_destroy.argtypes = [vp]
     for n in ("hmg_server_create", "hmg_server_submit", "hmg_server_poll",
               "hmg_server_job_times", "hmg_server_wait_all", "hmg_server_stats",
-              "hmg_server_destroy"):
+              "hmg_server_destroy", "hmg_server_set_max_active_cells"):
         getattr(L, n).restype = i32
     L._serve_bound = True
     return L
 
 
 class Server:
-    def __init__(self, storage=Storage.BF16, hbm_budget=0, params=None, ctx=None, **kw):
+    def __init__(self, storage=Storage.BF16, hbm_budget=0, params=None, ctx=None,
+                 max_active_cells=0, **kw):
         L = _bind()
         self.ctx = _ctx(ctx)
         self.params = params if params is not None else VpipesParams.make(**kw)
@@ -61,6 +63,8 @@ class Server:
                                    C.byref(self.params), C.byref(h)))
         self.handle = h
         self._out = {}
+        if max_active_cells:
+            _check(L.hmg_server_set_max_active_cells(h, int(max_active_cells)))
 
     def submit(self, nx, ny, kind, iterations, talus, seed=0, z=None, thermal_every=10):
         d = JobDesc(nx, ny, kind, iterations, thermal_every, talus, 0 if z is None else 1, seed)
@@ -173,46 +177,54 @@ def bench(args):
     sv.wait_all()
     sv.close()
 
-    sv = Server(storage=storage, params=params)
-    jobs = {}
-    lock = threading.Lock()
+    def served(cap):
+        sv = Server(storage=storage, params=params, max_active_cells=cap)
+        jobs = {}
+        lock = threading.Lock()
 
-    def submitter():
-        t0 = time.monotonic()
-        for r in reqs:
-            dt = t0 + r["arrival_s"] - time.monotonic()
-            if dt > 0:
-                time.sleep(dt)
-            kind = JOB_VPIPES_THERMAL if r["kind"] == "vpipes+thermal" else JOB_THERMAL
-            jid = sv.submit(r["side"], r["side"], kind, r["iterations"], job_talus(kind, r["side"]),
-                            seed=r["seed"])
+        def submitter():
+            t0 = time.monotonic()
+            for r in reqs:
+                dt = t0 + r["arrival_s"] - time.monotonic()
+                if dt > 0:
+                    time.sleep(dt)
+                kind = JOB_VPIPES_THERMAL if r["kind"] == "vpipes+thermal" else JOB_THERMAL
+                jid = sv.submit(r["side"], r["side"], kind, r["iterations"],
+                                job_talus(kind, r["side"]), seed=r["seed"])
+                with lock:
+                    jobs[r["id"]] = jid
+
+        th = threading.Thread(target=submitter)
+        wall0 = time.monotonic()
+        th.start()
+        done = {}
+        kept = {}
+        while len(done) < len(reqs):
             with lock:
-                jobs[r["id"]] = jid
+                pending = [(i, j) for i, j in jobs.items() if i not in done]
+            for i, j in pending:
+                st = sv.poll(j)
+                if st == DONE or st < 0:
+                    z = sv.result(j) if st == DONE else None
+                    if i in check_ids:
+                        kept[i] = z
+                    done[i] = sv.times(j)
+            time.sleep(0.0005)
+        th.join()
+        wall = time.monotonic() - wall0
+        stats = sv.stats()
+        sv.close()
+        lat = [t[2] - t[0] for t in done.values()]
+        first_submit = min(t[0] for t in done.values())
+        last_done = max(t[2] for t in done.values())
+        return {"rate": cell_iters / (last_done - first_submit) / 1e9, "lat": lat, "wall": wall,
+                "stats": stats, "kept": kept}
 
-    th = threading.Thread(target=submitter)
-    wall0 = time.monotonic()
-    th.start()
-    done = {}
-    kept = {}
-    while len(done) < len(reqs):
-        with lock:
-            pending = [(i, j) for i, j in jobs.items() if i not in done]
-        for i, j in pending:
-            st = sv.poll(j)
-            if st == DONE or st < 0:
-                z = sv.result(j) if st == DONE else None
-                if i in check_ids:
-                    kept[i] = z
-                done[i] = sv.times(j)
-        time.sleep(0.0005)
-    th.join()
-    wall = time.monotonic() - wall0
-    stats = sv.stats()
-    sv.close()
-    lat = [t[2] - t[0] for t in done.values()]
-    first_submit = min(t[0] for t in done.values())
-    last_done = max(t[2] for t in done.values())
-    served_rate = cell_iters / (last_done - first_submit) / 1e9
+    cap = int(getattr(args, "max_active_cells", 0) or 0)
+    main = served(cap)
+    other = served(0) if cap else None
+    kept = main["kept"]
+    lat, wall, stats, served_rate = main["lat"], main["wall"], main["stats"], main["rate"]
 
     # the same requests as sequential plain calls, each alone
     service = []
@@ -251,4 +263,9 @@ def bench(args):
                              "p99_latency_s": round(_pct(seq_lat, 99), 4),
                              "total_service_s": round(sum(service), 3)},
         "alone_equals_mix": same,
+        "max_active_cells": cap,
+        "uncapped": None if other is None else {
+            "gcell_iter_s": round(other["rate"], 3),
+            "p50_latency_s": round(_pct(other["lat"], 50), 4),
+            "p99_latency_s": round(_pct(other["lat"], 99), 4)},
     }

@@ -4,7 +4,10 @@
 //   hmap::gpu::init_opencl        HighMap/include/highmap/opencl/gpu_opencl.hpp:30
 //   hmap::gpu::noise_fbm          HighMap/include/highmap/primitives/coherent_noise.hpp:821-834
 //   hmap::gpu::thermal (3 overloads)  HighMap/include/highmap/erosion.hpp:1784-1801
+//   hmap::gpu::thermal_auto_bedrock (3) HighMap/include/highmap/erosion.hpp:1824-1838
+//   hmap::gpu::sediment_deposition (2)  HighMap/include/highmap/erosion.hpp:1366-1379
 //   hmap::gpu::hydraulic_vpipes   HighMap/include/highmap/erosion.hpp:1206-1223
+//   hmap::gpu::flow_simulation    HighMap/include/highmap/hydrology/hydrology.hpp:875-882
 // plus the minimal hmap::Array (HighMap/include/highmap/array.hpp:31-44,
 // 236-239: float storage, shape {nx, ny}, index j*nx + i) and hmap::NoiseType
 // (HighMap/include/highmap/functions.hpp:66-81) those signatures use.
@@ -127,6 +130,37 @@ void thermal(Array &z,
              Array *p_bedrock = nullptr,
              Array *p_deposition_map = nullptr);
 
+void thermal_auto_bedrock(Array       &z,
+                          const Array &talus,
+                          int          iterations = 10,
+                          Array       *p_deposition_map = nullptr);
+
+void thermal_auto_bedrock(Array       &z,
+                          const Array *p_mask,
+                          const Array &talus,
+                          int          iterations = 10,
+                          Array       *p_deposition_map = nullptr);
+
+void thermal_auto_bedrock(Array &z,
+                          float  talus,
+                          int    iterations = 10,
+                          Array *p_deposition_map = nullptr);
+
+void sediment_deposition(Array       &z,
+                         Array       *p_mask,
+                         const Array &talus,
+                         Array       *p_deposition_map = nullptr,
+                         float        max_deposition = 0.01f,
+                         int          iterations = 5,
+                         int          thermal_subiterations = 10);
+
+void sediment_deposition(Array       &z,
+                         const Array &talus,
+                         Array       *p_deposition_map = nullptr,
+                         float        max_deposition = 0.01f,
+                         int          iterations = 5,
+                         int          thermal_subiterations = 10);
+
 void hydraulic_vpipes(Array &z,
                       float  water_height = 1e-2f,
                       bool   maintain_water_volume = true,
@@ -145,6 +179,17 @@ void hydraulic_vpipes(Array &z,
                       Array *p_sediment = nullptr,
                       Array *p_vel_u = nullptr,
                       Array *p_vel_v = nullptr);
+
+// The parameter is named water_depth in the reference's declaration and
+// water_height in its definition (flow_simulation.cpp:26-33).
+Array flow_simulation(const Array &z,
+                      float        water_depth,
+                      const Array &depth_map,
+                      int          iterations,
+                      float        dt = 0.5f,
+                      bool         flux_diffusion = true,
+                      float        flux_diffusion_strength = 0.01f,
+                      float        dry_out_ratio = 0.f);
 
 } // namespace gpu
 } // namespace hmap

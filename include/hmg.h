@@ -197,6 +197,52 @@ int hmg_thermal_dev(hmg_ctx     *ctx,
                     int          iterations,
                     const float *bedrock_dev);
 
+/* hmap::gpu::thermal_auto_bedrock, the three overloads
+ * (HighMap/src/erosion/thermal_gpu.cpp:98-146, kernel thermal.cl:102-145;
+ * defaults iterations 10, include/highmap/erosion.hpp:1824-1838): Jacobi
+ * steps in which each interior cell keeps a bedrock that starts at
+ * -FLT_MAX and becomes max(z0, z) after each update, z = max(max(z + exchange,
+ * z0), bedrock); then extrapolate_borders.  talus_host NULL means
+ * talus_scalar everywhere; deposition (max(z - z0, 0)) and mask (the
+ * p_mask overload) as in hmg_thermal. */
+int hmg_thermal_auto_bedrock(hmg_ctx     *ctx,
+                             float       *z_host,
+                             int          nx,
+                             int          ny,
+                             const float *talus_host,
+                             float        talus_scalar,
+                             int          iterations,
+                             float       *deposition_host,
+                             const float *mask_host);
+/* Device form, in place on z_dev (no deposition or mask). */
+int hmg_thermal_auto_bedrock_dev(hmg_ctx     *ctx,
+                                 float       *z_dev,
+                                 int          nx,
+                                 int          ny,
+                                 const float *talus_dev,
+                                 float        talus_scalar,
+                                 int          iterations);
+
+/* hmap::gpu::sediment_deposition, both overloads
+ * (HighMap/src/erosion/deposition.cpp:19-63; defaults max_deposition 0.01,
+ * iterations 5, thermal_subiterations 10, include/highmap/erosion.hpp:
+ * 1366-1379): `iterations` rounds of raising a sediment layer by
+ * max_deposition / iterations and letting it settle with thermal(z + layer,
+ * talus, thermal_subiterations); z += layer.  deposition_host (may be NULL)
+ * receives the layer; mask_host (may be NULL) blends z = lerp(z, result,
+ * mask). */
+int hmg_sediment_deposition(hmg_ctx     *ctx,
+                            float       *z_host,
+                            int          nx,
+                            int          ny,
+                            const float *talus_host,
+                            float        talus_scalar,
+                            float       *deposition_host,
+                            float        max_deposition,
+                            int          iterations,
+                            int          thermal_subiterations,
+                            const float *mask_host);
+
 /* ------------------------------------------------------------- vpipes -- */
 
 /* hmap::gpu::hydraulic_vpipes (HighMap/src/erosion/
@@ -213,6 +259,42 @@ int hmg_vpipes(hmg_ctx                 *ctx,
                float                   *s_out_host,
                float                   *u_out_host,
                float                   *v_out_host);
+
+/* ---------------------------------------------------- flow simulation -- */
+
+/* hmap::gpu::flow_simulation (HighMap/src/hydrology/flow_simulation.cpp:
+ * 26-110; defaults in include/highmap/hydrology/hydrology.hpp:875-882:
+ * dt 0.5, flux_diffusion 1, flux_diffusion_strength 0.01, dry_out_ratio 0).
+ * d = water_height * depth_map, then `iterations` flow + water passes of
+ * hydraulic_vpipes (no erosion, evaporation or top-up), then
+ * water_depth_dry_out(d, dry_out_ratio) when dry_out_ratio != 0.  Writes the
+ * water depth (nx*ny floats) to d_out_host; z and depth_map are not
+ * changed. */
+int hmg_flow_simulation(hmg_ctx     *ctx,
+                        const float *z_host,
+                        int          nx,
+                        int          ny,
+                        float        water_height,
+                        const float *depth_map_host,
+                        int          iterations,
+                        float        dt,
+                        int          flux_diffusion,
+                        float        flux_diffusion_strength,
+                        float        dry_out_ratio,
+                        float       *d_out_host);
+/* The same on device pointers (d_out_dev may not alias the inputs). */
+int hmg_flow_simulation_dev(hmg_ctx     *ctx,
+                            const float *z_dev,
+                            int          nx,
+                            int          ny,
+                            float        water_height,
+                            const float *depth_map_dev,
+                            int          iterations,
+                            float        dt,
+                            int          flux_diffusion,
+                            float        flux_diffusion_strength,
+                            float        dry_out_ratio,
+                            float       *d_out_dev);
 
 /* ------------------------------------------------------------ session -- */
 
@@ -353,6 +435,12 @@ int hmg_server_job_times(hmg_server *sv,
                          double     *t_submit,
                          double     *t_admit,
                          double     *t_done);
+/* Caps the cells (sum of nx*ny) of the jobs that run at once; admission
+ * waits while the cap is reached (a job larger than the cap still runs when
+ * nothing else does).  0, the default, means no cap.  Under overload a cap
+ * of a few full-device tiles keeps the device saturated while later jobs
+ * wait in arrival order instead of all sharing it. */
+int hmg_server_set_max_active_cells(hmg_server *sv, size_t cells);
 int hmg_server_wait_all(hmg_server *sv);
 int hmg_server_stats(hmg_server *sv,
                      double     *cell_iters,

@@ -9,6 +9,8 @@
  *                             hydraulic iteration (talus 1/nx)
  *             vpipes          vpipes only
  *             thermal         thermal only (talus 2/nx)
+ *             flow            flow_simulation (water_height 1e-2, uniform
+ *                             depth map, dt 0.5, flux diffusion 0.01)
  * Input: remap(noise_fbm(PERLIN, kw 4, seed 2)) to [0,1] (not timed).
  * Output: one JSON line with the median wall time of `reps` timed runs after
  * one warm-up run.
@@ -81,7 +83,13 @@ int main(int argc, char **argv)
     double W0, R;
     orc_vpipes_init_f32(d, s, fl, fr, ft, fb, u, v, nx, ny, &p, NULL, &W0, &R);
     double t0 = now_s();
-    if (!strcmp(wl, "thermal"))
+    if (!strcmp(wl, "flow"))
+    {
+      for (size_t k = 0; k < n; k++) u[k] = 1.f; /* depth map */
+      orc_flow_simulation_f32(z, nx, ny, 1e-2f, u, iters, 0.5f, 1, 0.01f, 0.f, d);
+      z[n / 2] = d[n / 2]; /* checksum below */
+    }
+    else if (!strcmp(wl, "thermal"))
     {
       orc_thermal_steps_f32(z, nx, ny, NULL, 2.f / (float)nx, iters);
       orc_extrapolate_borders_f32(z, nx, ny);

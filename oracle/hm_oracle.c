@@ -12,6 +12,7 @@
 #include "hm_oracle.h"
 
 #include <math.h>
+#include <float.h>
 #include <stdlib.h>
 #include <string.h>
 

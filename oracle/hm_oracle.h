@@ -114,6 +114,22 @@ void orc_thermal_steps_f64(double       *z,
 void orc_extrapolate_borders_f32(float *z, int nx, int ny);
 void orc_extrapolate_borders_f64(double *z, int nx, int ny);
 
+/* thermal_auto_bedrock Jacobi steps (no border extrapolation); talus may be
+ * NULL (talus_scalar everywhere). */
+void orc_thermal_auto_bedrock_steps_f32(float *z, int nx, int ny, const float *talus,
+                                        float talus_scalar, int iterations);
+void orc_thermal_auto_bedrock_steps_f64(double *z, int nx, int ny, const double *talus,
+                                        double talus_scalar, int iterations);
+
+/* hmap::gpu::sediment_deposition without a mask; z updated in place. */
+void orc_sediment_deposition_f32(float *z, int nx, int ny, const float *talus,
+                                 float talus_scalar, float *deposition, float max_deposition,
+                                 int iterations, int thermal_subiterations);
+void orc_sediment_deposition_f64(double *z, int nx, int ny, const double *talus,
+                                 double talus_scalar, double *deposition,
+                                 float max_deposition, int iterations,
+                                 int thermal_subiterations);
+
 /* --- fixed-point water volume (order-independent, exact per term) --- */
 double orc_fixsum_f32(const float *a, size_t n);
 void   orc_fixsum_parts(const float *a, size_t n, uint64_t *lo, int64_t *hi);
@@ -184,6 +200,17 @@ void orc_vpipes_run_f64(double        *z,
                         double         W0,
                         double         R,
                         int            iterations);
+
+/* hmap::gpu::flow_simulation: the water depth after `iterations` flow and
+ * water passes (and the optional dry-out) into d (nx*ny). */
+void orc_flow_simulation_f32(const float *z, int nx, int ny, float water_height,
+                             const float *depth_map, int iterations, float dt,
+                             int flux_diffusion, float flux_diffusion_strength,
+                             float dry_out_ratio, float *d);
+void orc_flow_simulation_f64(const double *z, int nx, int ny, float water_height,
+                             const double *depth_map, int iterations, float dt,
+                             int flux_diffusion, float flux_diffusion_strength,
+                             float dry_out_ratio, double *d);
 
 /* Single-pass entry points, for pass-level tests. */
 void orc_vp_flow_f32(const float *z, const float *d1, const float *fl,

@@ -125,6 +125,19 @@ def lib():
     L.orc_vp_water_f32.argtypes = [F32P] * 8 + [C.c_int, C.c_int, pp]
     L.orc_vp_erosion_f32.argtypes = [F32P] * 7 + [C.c_int, C.c_int, pp]
     L.orc_vp_transport_f32.argtypes = [F32P] * 4 + [C.c_int, C.c_int, C.c_float]
+    L.orc_thermal_auto_bedrock_steps_f32.argtypes = [F32P, C.c_int, C.c_int, C.c_void_p,
+                                                     C.c_float, C.c_int]
+    L.orc_thermal_auto_bedrock_steps_f64.argtypes = [F64P, C.c_int, C.c_int, C.c_void_p,
+                                                     C.c_double, C.c_int]
+    L.orc_sediment_deposition_f32.argtypes = [F32P, C.c_int, C.c_int, C.c_void_p, C.c_float,
+                                              C.c_void_p, C.c_float, C.c_int, C.c_int]
+    L.orc_sediment_deposition_f64.argtypes = [F64P, C.c_int, C.c_int, C.c_void_p, C.c_double,
+                                              C.c_void_p, C.c_float, C.c_int, C.c_int]
+    fl = C.c_float
+    L.orc_flow_simulation_f32.argtypes = [F32P, C.c_int, C.c_int, fl, F32P, C.c_int, fl, C.c_int,
+                                          fl, fl, F32P]
+    L.orc_flow_simulation_f64.argtypes = [F64P, C.c_int, C.c_int, fl, F64P, C.c_int, fl, C.c_int,
+                                          fl, fl, F64P]
     _lib = L
     return L
 
@@ -201,6 +214,41 @@ def thermal(z, talus, iterations=10, dtype=np.float32):
     return extrapolate_borders(thermal_steps(z, talus, iterations, dtype), dtype)
 
 
+def _talus_args(talus, shape, dtype):
+    if np.isscalar(talus):
+        return None, float(talus)
+    t = np.ascontiguousarray(talus, dtype)
+    if t.shape != shape:
+        raise ValueError("talus shape mismatch")
+    return t, 0.0
+
+
+def thermal_auto_bedrock(z, talus, iterations=10, dtype=np.float32):
+    """hmap::gpu::thermal_auto_bedrock restated: Jacobi steps then
+    extrapolate_borders; returns the new z."""
+    z = np.ascontiguousarray(z, dtype).copy()
+    ny, nx = z.shape
+    t, ts = _talus_args(talus, z.shape, dtype)
+    fn = (lib().orc_thermal_auto_bedrock_steps_f32 if dtype == np.float32
+          else lib().orc_thermal_auto_bedrock_steps_f64)
+    fn(z, nx, ny, _ptr(t), ts, int(iterations))
+    return extrapolate_borders(z, dtype)
+
+
+def sediment_deposition(z, talus, max_deposition=0.01, iterations=5, thermal_subiterations=10,
+                        dtype=np.float32):
+    """hmap::gpu::sediment_deposition restated; returns (z, deposition)."""
+    z = np.ascontiguousarray(z, dtype).copy()
+    ny, nx = z.shape
+    t, ts = _talus_args(talus, z.shape, dtype)
+    dep = np.empty_like(z)
+    fn = (lib().orc_sediment_deposition_f32 if dtype == np.float32
+          else lib().orc_sediment_deposition_f64)
+    fn(z, nx, ny, _ptr(t), ts, _ptr(dep), float(max_deposition), int(iterations),
+       int(thermal_subiterations))
+    return z, dep
+
+
 def fixsum(a):
     a = np.ascontiguousarray(a, np.float32).reshape(-1)
     return lib().orc_fixsum_f32(a, a.size)
@@ -257,3 +305,21 @@ def fixsum_parts(a):
     hi = C.c_int64()
     lib().orc_fixsum_parts(a, a.size, C.byref(lo), C.byref(hi))
     return (hi.value << 64) + lo.value
+
+
+# ------------------------------------------------------- flow simulation
+
+
+def flow_simulation(z, water_height, depth_map, iterations, dt=0.5, flux_diffusion=True,
+                    flux_diffusion_strength=0.01, dry_out_ratio=0.0, dtype=np.float32):
+    """hmap::gpu::flow_simulation restated; returns the water depth."""
+    z = np.ascontiguousarray(z, dtype)
+    ny, nx = z.shape
+    dm = np.ascontiguousarray(depth_map, dtype)
+    if dm.shape != z.shape:
+        raise ValueError("depth_map shape mismatch")
+    d = np.empty_like(z)
+    fn = lib().orc_flow_simulation_f32 if dtype == np.float32 else lib().orc_flow_simulation_f64
+    fn(z, nx, ny, float(water_height), dm, int(iterations), float(dt), int(bool(flux_diffusion)),
+       float(flux_diffusion_strength), float(dry_out_ratio), d)
+    return d

@@ -1,6 +1,9 @@
 // shim_driver.cpp -- calls the hmap::gpu drop-in exactly as HighMap's own
 // examples do (examples/ex_thermal/ex_thermal.cpp:7-16,
-// examples/ex_hydraulic_vpipes_gpu/ex_hydraulic_vpipes_gpu.cpp:7-22) and
+// examples/ex_hydraulic_vpipes_gpu/ex_hydraulic_vpipes_gpu.cpp:7-22,
+// examples/ex_flow_simulation/ex_flow_simulation.cpp:5-26,
+// examples/ex_thermal_auto_bedrock/ex_thermal_auto_bedrock.cpp:5-15,
+// examples/ex_sediment_deposition/ex_sediment_deposition.cpp:5-18) and
 // writes every result as raw float32 into the directory given, for the GPU
 // test to compare with the oracle.
 #include <cstdio>
@@ -74,6 +77,25 @@ int main(int argc, char **argv)
   }
   hmap::gpu::hydraulic_vpipes(h2);
   dump(dir, "vp_default_z", h2);
+
+  // flow_simulation, as ex_flow_simulation (its CPU noise and bulkify are
+  // replaced by the GPU noise of seed 2, remapped)
+  hmap::Array zf = hmap::gpu::noise_fbm(hmap::NoiseType::PERLIN, shape, {4.f, 4.f}, 2, 8, 0.7f);
+  hmap::remap(zf);
+  hmap::Array water_depth = hmap::gpu::flow_simulation(zf, 1e-2f, hmap::Array(shape, 1.f), 400);
+  dump(dir, "flow256", water_depth);
+
+  // thermal_auto_bedrock and sediment_deposition, as their examples (GPU
+  // noise in place of the CPU noise)
+  hmap::Array za = hmap::gpu::noise_fbm(hmap::NoiseType::PERLIN, shape, {2.f, 2.f}, 1);
+  dump(dir, "auto_in", za);
+  hmap::gpu::thermal_auto_bedrock(za, 0.5f / shape.x, 500);
+  dump(dir, "auto256", za);
+  hmap::Array zs = hmap::gpu::noise_fbm(hmap::NoiseType::PERLIN, shape, {4.f, 4.f}, 1);
+  hmap::Array talus_s(shape, 2.f / (float)shape.x), dep_s;
+  hmap::gpu::sediment_deposition(zs, talus_s, &dep_s, 0.01f, 10, 50);
+  dump(dir, "sed256", zs);
+  dump(dir, "sed_dep256", dep_s);
   std::printf("shim_driver ok\n");
   return 0;
 }

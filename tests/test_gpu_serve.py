@@ -31,13 +31,15 @@ def test_job_larger_than_budget_is_rejected(hm):
 
 
 # budget 0: 80% of free HBM (everything admitted at once); 6 MiB: the
-# largest job (320^2) fits alone, so admission throttles the mix
-@pytest.mark.parametrize("budget", [0, 6 << 20])
-def test_served_jobs_equal_jobs_alone(hm, budget):
+# largest job (320^2) fits alone, so admission throttles the mix; a cap of
+# 200^2 active cells runs the larger jobs alone and the small ones in pairs
+@pytest.mark.parametrize("budget,cap", [(0, 0), (6 << 20, 0), (0, 200 * 200)])
+def test_served_jobs_equal_jobs_alone(hm, budget, cap):
     from highmap_amd import serve
 
     p = _params(hm)
-    sv = serve.Server(storage=hm.Storage.BF16, hbm_budget=budget, params=p)
+    sv = serve.Server(storage=hm.Storage.BF16, hbm_budget=budget, params=p,
+                      max_active_cells=cap)
     ids = []
     for r in REQS:
         kind = serve.JOB_VPIPES_THERMAL if r["kind"] == "vpipes+thermal" else serve.JOB_THERMAL
@@ -52,6 +54,10 @@ def test_served_jobs_equal_jobs_alone(hm, budget):
         assert t_sub <= t_adm <= t_done
     st = sv.stats()
     assert st["used"] == 0 and st["ticks"] >= 1
+    if cap:
+        # arrival order: no job is admitted before an earlier one
+        adm = [sv.times(j)[1] for j in ids]
+        assert adm == sorted(adm)
     sv.close()
 
 

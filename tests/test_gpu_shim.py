@@ -58,3 +58,24 @@ def test_shim_vpipes(out, orc):
         assert np.array_equal(out(name, shp), plane), name
     dft = orc.VpState(zin, orc.vp_params()).run(50)
     assert np.array_equal(out("vp_default_z", shp), dft.z)
+
+
+def test_shim_flow_simulation(out, orc):
+    z = orc.remap(orc.noise_fbm(256, 256, (4.0, 4.0), 2))
+    want = orc.flow_simulation(z, 1e-2, np.ones_like(z), 400)
+    assert np.array_equal(out("flow256", (256, 256)), want)
+
+
+def test_shim_thermal_auto_bedrock(out, orc):
+    zin = out("auto_in", (256, 256))
+    assert np.array_equal(zin, orc.noise_fbm(256, 256, (2.0, 2.0), 1))
+    want = orc.thermal_auto_bedrock(zin, np.float32(0.5) / np.float32(256), 500)
+    assert np.array_equal(out("auto256", (256, 256)), want)
+
+
+def test_shim_sediment_deposition(out, orc):
+    z0 = orc.noise_fbm(256, 256, (4.0, 4.0), 1)
+    talus = np.full((256, 256), np.float32(2.0) / np.float32(256.0), np.float32)
+    z, dep = orc.sediment_deposition(z0, talus, 0.01, 10, 50)
+    assert np.array_equal(out("sed256", (256, 256)), z)
+    assert np.array_equal(out("sed_dep256", (256, 256)), dep)
