@@ -124,6 +124,8 @@ def run_headline(args):
     stream = torch.cuda.ExternalStream(ctx.stream)
     z = device_terrain(torch, hm, ctx, n, n, 2)
     sess = hm.Session(z, storage=STORAGE[args.storage], mode=args.mode, **WORKLOAD)
+    if args.chunk_rows:
+        sess.set_tuning(args.chunk_rows, args.prefetch)
     talus = 1.0 / n
 
     def step():
@@ -159,6 +161,7 @@ def run_headline(args):
     launch_s = e0.elapsed_time(e1) / 1e3 / m
     alg = ALG_BYTES[args.storage] * n * n
     achieved = alg / launch_s / 1e9
+    chunk_rows, prefetch, _ = sess.tuning()
     sess.close()
 
     kernel = "k_vp_stream" if args.mode == "fused" else "k_vp_* (plain passes)"
@@ -184,6 +187,8 @@ def run_headline(args):
             "params": WORKLOAD,
             "storage": args.storage,
             "mode": args.mode,
+            "launch_shape": {"chunk_rows": chunk_rows, "prefetch": prefetch,
+                             "source": "set" if args.chunk_rows else "online tuning"},
         },
         "pct_hbm_peak_step": round(100.0 * (ALG_BYTES[args.storage] * n * n * iters +
                                             8 * n * n * args.steps) / elapsed / 1e9
@@ -524,6 +529,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--storage", default="f32", choices=sorted(STORAGE))
     ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
+    ap.add_argument("--chunk-rows", type=int, default=0,
+                    help="headline: fixed rows per wavefront (0: online tuning)")
+    ap.add_argument("--prefetch", type=int, default=1, choices=[0, 1])
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--iters", type=int, default=500)
     ap.add_argument("--maps", type=int, default=0)

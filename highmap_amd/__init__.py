@@ -127,6 +127,8 @@ def lib():
             "hmg_noise_fbm_dev": ([vp, vp, i32, i32, i32, f32, f32, u32, i32, f32, f32, f32,
                                    vp, vp, vp, vp, vp], i32),
             "hmg_remap_dev": ([vp, vp, sz, f32, f32], i32),
+            "hmg_noise": ([vp, vp, i32, i32, i32, f32, f32, u32, vp, vp, vp, vp], i32),
+            "hmg_noise_dev": ([vp, vp, i32, i32, i32, f32, f32, u32, vp, vp, vp, vp], i32),
             "hmg_thermal": ([vp, vp, i32, i32, vp, f32, i32, vp, vp, vp], i32),
             "hmg_thermal_dev": ([vp, vp, i32, i32, vp, f32, i32, vp], i32),
             "hmg_vpipes": ([vp, vp, i32, i32, pp, vp, vp, vp, vp, vp], i32),
@@ -139,6 +141,8 @@ def lib():
                                   C.POINTER(C.c_double)], i32),
             "hmg_session_z_dev": ([vp], vp),
             "hmg_session_set_tuning": ([vp, i32, i32], i32),
+            "hmg_session_get_tuning": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)],
+                                       i32),
             "hmg_session_sync": ([vp], i32),
             "hmg_session_destroy": ([vp], i32),
             "hmg_session_bytes": ([i32, i32, i32], sz),
@@ -255,6 +259,22 @@ def noise_fbm(noise_type, shape, kw, seed, octaves=8, weight=0.7, persistence=0.
                                float(kw[0]), float(kw[1]), int(seed) & 0xFFFFFFFF, int(octaves),
                                float(weight), float(persistence), float(lacunarity),
                                _ptr(c), _ptr(gx), _ptr(gy), _ptr(bb), _ptr(pe)))
+    return out
+
+
+def noise(noise_type, shape, kw, seed, noise_x=None, noise_y=None, stretching=None,
+          bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0), ctx=None):
+    """hmap::gpu::noise (one octave); returns a float32 array of shape
+    (ny, nx).  `stretching` is accepted and ignored, as in the reference."""
+    nx, ny = int(shape[0]), int(shape[1])
+    out = np.empty((ny, nx), np.float32)
+    gx = _f32(noise_x, (ny, nx), "noise_x")
+    gy = _f32(noise_y, (ny, nx), "noise_y")
+    bb = np.asarray(bbox, np.float32)
+    pe = np.asarray(period, np.int32)
+    _check(lib().hmg_noise(_ctx(ctx).handle, _ptr(out), nx, ny, int(noise_type), float(kw[0]),
+                           float(kw[1]), int(seed) & 0xFFFFFFFF, _ptr(gx), _ptr(gy), _ptr(bb),
+                           _ptr(pe)))
     return out
 
 
@@ -424,6 +444,12 @@ class Session:
     def set_tuning(self, chunk_rows, prefetch=1):
         _check(lib().hmg_session_set_tuning(self.handle, int(chunk_rows), int(prefetch)))
         return self
+
+    def tuning(self):
+        """(chunk_rows, prefetch, still_tuning) of the fused kernel."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().hmg_session_get_tuning(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, bool(c.value)
 
     def z_dev_ptr(self):
         return lib().hmg_session_z_dev(self.handle)

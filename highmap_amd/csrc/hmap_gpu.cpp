@@ -82,6 +82,38 @@ bool init_opencl()
   return hmg_init(0, &g_ctx) == HMG_OK;
 }
 
+// HighMap/src/primitives/coherent_noise/primitives_gpu.cpp:504-541
+Array noise(NoiseType     noise_type,
+            glm::ivec2    shape,
+            glm::vec2     kw,
+            std::uint32_t seed,
+            const Array  *p_noise_x,
+            const Array  *p_noise_y,
+            const Array * /* p_stretching: ignored, as in the reference */,
+            glm::vec4     bbox,
+            glm::ivec2    period)
+{
+  Array out(shape);
+  if (p_noise_x) check_shape(*p_noise_x, shape, "noise_x");
+  if (p_noise_y) check_shape(*p_noise_y, shape, "noise_y");
+  const float bb[4] = {bbox.x, bbox.y, bbox.z, bbox.w};
+  const int   per[2] = {period.x, period.y};
+  check(hmg_noise(ctx(),
+                  out.vector.data(),
+                  shape.x,
+                  shape.y,
+                  (int)noise_type,
+                  kw.x,
+                  kw.y,
+                  seed,
+                  p_noise_x ? p_noise_x->vector.data() : nullptr,
+                  p_noise_y ? p_noise_y->vector.data() : nullptr,
+                  bb,
+                  per),
+        "noise");
+  return out;
+}
+
 Array noise_fbm(NoiseType     noise_type,
                 glm::ivec2    shape,
                 glm::vec2     kw,

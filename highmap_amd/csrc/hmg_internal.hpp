@@ -157,6 +157,16 @@ struct hmg_session
   float             rf = 0.f;
   int               chunk_rows = 64;
   int               prefetch = 1;
+  // Online launch-shape tuning of the fused kernel (vpipes.hip): the first
+  // hydraulic iterations of a large session each run one candidate shape
+  // between two events; the fastest shape is then kept (and cached for the
+  // process).  Results do not depend on the shape.  tune_pos < 0: done.
+  int                     tune_pos = -1;
+  std::vector<hipEvent_t> tune_ev;
+  ~hmg_session()
+  {
+    for (hipEvent_t e : tune_ev) (void)hipEventDestroy(e);
+  }
 };
 
 namespace hmg
@@ -175,6 +185,7 @@ int batch_advance(hmg_ctx           *ctx,
 int launch_noise_fbm(float       *out,
                      int          nx,
                      int          ny,
+                     int          noise_type,
                      float        kx,
                      float        ky,
                      uint32_t     seed,

@@ -1,15 +1,23 @@
-// noise.hip -- hmap::gpu::noise_fbm, PERLIN branch, for gfx950.
+// noise.hip -- hmap::gpu::noise_fbm and hmap::gpu::noise for gfx950.
 //
-// Follows the reference kernel noise_fbm (HighMap/src/gpu_opencl/kernels/
-// noise.cl:456-504) with base_perlin_fbm (:194-216), base_perlin (:28-63),
-// wrap_lattice/scale_period (:12-26), g_to_xy (_common_index.cl:8-24) and
-// the hashes of _common_rand.cl; host wrapper
-// HighMap/src/primitives/coherent_noise/primitives_gpu.cpp:543-591.
+// Follows the reference kernels noise_fbm and noise
+// (HighMap/src/gpu_opencl/kernels/noise.cl:385-577) with the base noises
+// (:28-190), their fBm loops (:194-383), wrap_lattice/scale_period (:12-26),
+// g_to_xy (_common_index.cl:8-24), the hashes of _common_rand.cl and the
+// helpers of _common_math.cl; host wrappers
+// HighMap/src/primitives/coherent_noise/primitives_gpu.cpp:504-591.
+// Implemented types (hmap::NoiseType, functions.hpp:66-81): PERLIN 1,
+// PERLIN_BILLOW 2, PERLIN_HALF 3, SIMPLEX2 4, VALUE 6, VALUE_CUBIC 7,
+// VALUE_LINEAR 9, WORLEY 10; the kernels write 0 for the other types and so
+// does this file.
 //
-// The kernel is VALU-bound (8 octaves x 4 corners x one hash sin and one
-// sincos), not memory-bound: one 4-byte store per cell.  sin/cos are the
-// project-written routines of hmg_cell.hpp, so the field matches the FP32
-// restatement bit for bit.
+// The kernels are VALU-bound (hash sines and sincos per lattice corner per
+// octave), not memory-bound: one 4-byte store per cell.  sin/cos are the
+// project-written routines of hmg_cell.hpp, so each field matches the FP32
+// restatement bit for bit.  One kernel instance per type (template), so the
+// per-cell work carries no type dispatch.
+#include <cfloat>
+
 #include "hmg_internal.hpp"
 
 namespace hmg
@@ -62,6 +70,129 @@ __device__ __forceinline__ float base_perlin(float px, float py, float fseed, in
   return 1.42857f * lerp_cl(a, b, uy);
 }
 
+// Simplex noise on the skewed triangular lattice: the input is halved, the
+// cell found with the skew (sqrt(3)-1)/2 and unskew (3-sqrt(3))/6 constants,
+// and each of the three vertices adds (0.5 - |d|^2)^4 (g . d); scale 100.
+__device__ __forceinline__ float base_simplex2(float px, float py, float fseed)
+{
+  constexpr float K1 = 0.366025403f, K2 = 0.211324865f;
+  const float     x = 0.5f * px, y = 0.5f * py;
+  const float     skew = x * K1 + y * K1;
+  const float     ix = floorf(x + skew), iy = floorf(y + skew);
+  const float     unskew = ix * K2 + iy * K2;
+  const float     ax = (x - ix) + unskew, ay = (y - iy) + unskew;
+  const float     m = ax < ay ? 0.f : 1.f; // second vertex: +x or +y
+  const float     ox = m, oy = 1.f - m;
+  const float     vx[3] = {ax, (ax - ox) + K2, (ax - 1.f) + 2.f * K2};
+  const float     vy[3] = {ay, (ay - oy) + K2, (ay - 1.f) + 2.f * K2};
+  const float     lx[3] = {ix, ix + ox, ix + 1.f}, ly[3] = {iy, iy + oy, iy + 1.f};
+  float           n[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+  {
+    const float w = fmaxf(0.5f - (vx[k] * vx[k] + vy[k] * vy[k]), 0.f);
+    n[k] = w * w * w * w * perlin_corner(lx[k], ly[k], fseed, vx[k], vy[k]);
+  }
+  return 100.f * (n[0] + n[1] + n[2]);
+}
+
+// Value noise: bilinear blend of the four lattice hashes, with the cubic
+// fade x^2 (3 - 2x) (VALUE) or linear weights (VALUE_LINEAR), to [-1, 1].
+template <bool FADE>
+__device__ __forceinline__ float base_value(float px, float py, float fseed, int per_x, int per_y)
+{
+  const float ix = floorf(px), iy = floorf(py);
+  const float fx = fract_cl(px), fy = fract_cl(py);
+  const float x0 = wrap_lattice(ix, per_x), x1 = wrap_lattice(ix + 1.f, per_x);
+  const float y0 = wrap_lattice(iy, per_y), y1 = wrap_lattice(iy + 1.f, per_y);
+  const float v00 = hash12f(x0, y0, fseed), v10 = hash12f(x1, y0, fseed);
+  const float v01 = hash12f(x0, y1, fseed), v11 = hash12f(x1, y1, fseed);
+  const float ux = FADE ? fx * fx * (3.f - 2.f * fx) : fx;
+  const float uy = FADE ? fy * fy * (3.f - 2.f * fy) : fy;
+  return 2.f * lerp_cl(lerp_cl(v00, v10, ux), lerp_cl(v01, v11, ux), uy) - 1.f;
+}
+
+// Cubic through four samples at t in [0, 1] between p1 and p2, evaluated
+// left to right as a t^3 + b t^2 + c t + d (cubic_interp,
+// _common_math.cl:30-37).
+__device__ __forceinline__ float cubic4(float p0, float p1, float p2, float p3, float t)
+{
+  const float a = p3 - p2 - p0 + p1;
+  const float b = p0 - p1 - a;
+  const float c = p2 - p0;
+  return a * t * t * t + b * t * t + c * t + p1;
+}
+
+// VALUE_CUBIC: 4x4 lattice hashes, cubic along x per row, then along y.
+__device__ __forceinline__ float base_value_cubic(float px, float py, float fseed, int per_x,
+                                                  int per_y)
+{
+  const float ix = floorf(px), iy = floorf(py);
+  const float fx = fract_cl(px), fy = fract_cl(py);
+  float       row[4];
+#pragma unroll
+  for (int dy = 0; dy < 4; dy++)
+  {
+    const float yy = wrap_lattice(iy + (float)(dy - 1), per_y);
+    float       v[4];
+#pragma unroll
+    for (int dx = 0; dx < 4; dx++) v[dx] = hash12f(wrap_lattice(ix + (float)(dx - 1), per_x), yy, fseed);
+    row[dy] = cubic4(v[0], v[1], v[2], v[3], fx);
+  }
+  return 1.43f * (cubic4(row[0], row[1], row[2], row[3], fy) - 0.5f);
+}
+
+// hash22f (_common_rand.cl:28-34).
+__device__ __forceinline__ void hash22f(float px, float py, float fseed, float &hx, float &hy)
+{
+  const float qx = px * 127.1f + py * 311.7f;
+  const float qy = px * 269.5f + py * 183.3f;
+  hx = fract_cl(hm_sin(qx + fseed) * 43758.5453123f);
+  hy = fract_cl(hm_sin(qy + fseed) * 43758.5453123f);
+}
+
+// WORLEY: squared distance to the nearest of the feature points of the 3x3
+// surrounding cells (x outer, y inner, as the reference scans), 1.66 d^2 - 1.
+__device__ __forceinline__ float base_worley(float px, float py, float fseed, int per_x, int per_y)
+{
+  const float ix = floorf(px), iy = floorf(py);
+  const float fx = fract_cl(px), fy = fract_cl(py);
+  float       dmin = FLT_MAX;
+#pragma unroll
+  for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+    {
+      float hx, hy;
+      hash22f(wrap_lattice(ix + (float)dx, per_x), wrap_lattice(iy + (float)dy, per_y), fseed, hx,
+              hy);
+      const float ex = fx - ((float)dx + hx), ey = fy - ((float)dy + hy);
+      dmin = fminf(dmin, ex * ex + ey * ey);
+    }
+  return 1.66f * dmin - 1.f;
+}
+
+// max_smooth(a, b, k) (_common_math.cl:52-56), pow_int(h, 3) as ((1 h) h) h.
+__device__ __forceinline__ float max_smooth(float a, float b, float k)
+{
+  const float h = fmaxf(k - fabsf(a - b), 0.f) / k;
+  return fmaxf(a, b) + h * h * h * k / 6.f;
+}
+
+template <int TYPE>
+__device__ __forceinline__ float base_noise(float px, float py, float fseed, int per_x, int per_y)
+{
+  if constexpr (TYPE == 1) return base_perlin(px, py, fseed, per_x, per_y);
+  if constexpr (TYPE == 2) return 2.f * fabsf(base_perlin(px, py, fseed, per_x, per_y)) - 1.f;
+  if constexpr (TYPE == 3) return max_smooth(base_perlin(px, py, fseed, per_x, per_y), 0.f, 0.5f);
+  if constexpr (TYPE == 4) return base_simplex2(px, py, fseed);
+  if constexpr (TYPE == 6) return base_value<true>(px, py, fseed, per_x, per_y);
+  if constexpr (TYPE == 7) return base_value_cubic(px, py, fseed, per_x, per_y);
+  if constexpr (TYPE == 9) return base_value<false>(px, py, fseed, per_x, per_y);
+  if constexpr (TYPE == 10) return base_worley(px, py, fseed, per_x, per_y);
+  return 0.f;
+}
+
 struct NoiseArgs
 {
   int   nx, ny;
@@ -73,31 +204,37 @@ struct NoiseArgs
   int   per_x, per_y;
 };
 
-__global__ void __launch_bounds__(256) k_noise_fbm_perlin(float *__restrict__ out,
-                                                          const float *__restrict__ ctrl,
-                                                          const float *__restrict__ nxp,
-                                                          const float *__restrict__ nyp,
-                                                          NoiseArgs a)
+// g_to_xy (_common_index.cl:8-24)
+__device__ __forceinline__ void noise_xy(int i, int j, const NoiseArgs &a, float dx, float dy,
+                                         float &x, float &y)
+{
+  x = (float)i / (float)a.nx;
+  y = (float)j / (float)a.ny;
+  x = a.kx * (x * (a.b1 - a.b0) + a.b0) + a.kx * dx;
+  y = a.ky * (y * (a.b3 - a.b2) + a.b2) + a.ky * dy;
+}
+
+template <int TYPE>
+__global__ void __launch_bounds__(256) k_noise_fbm(float *__restrict__ out,
+                                                   const float *__restrict__ ctrl,
+                                                   const float *__restrict__ nxp,
+                                                   const float *__restrict__ nyp,
+                                                   NoiseArgs a)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= a.nx || j >= a.ny) return;
   size_t idx = (size_t)j * a.nx + i;
   float  ct = ctrl ? ctrl[idx] : 1.f;
-  float  dx = nxp ? nxp[idx] : 0.f;
-  float  dy = nyp ? nyp[idx] : 0.f;
   float  w = (1.f - ct) + ct * a.weight;
-
-  float x = (float)i / (float)a.nx;
-  float y = (float)j / (float)a.ny;
-  x = a.kx * (x * (a.b1 - a.b0) + a.b0) + a.kx * dx;
-  y = a.ky * (y * (a.b3 - a.b2) + a.b2) + a.ky * dy;
+  float  x, y;
+  noise_xy(i, j, a, nxp ? nxp[idx] : 0.f, nyp ? nyp[idx] : 0.f, x, y);
 
   float n = 0.f, nf = 1.f, na = 0.6f;
   int   px = a.per_x, py = a.per_y;
   for (int k = 0; k < a.octaves; k++)
   {
-    float v = base_perlin(x * nf, y * nf, a.fseed, px, py);
+    float v = base_noise<TYPE>(x * nf, y * nf, a.fseed, px, py);
     n += v * na;
     na *= (1.f - w) + w * fminf(v + 1.f, 2.f) * 0.5f;
     na *= a.persistence;
@@ -108,22 +245,32 @@ __global__ void __launch_bounds__(256) k_noise_fbm_perlin(float *__restrict__ ou
   out[idx] = n;
 }
 
-int launch_noise_fbm(float       *out,
-                     int          nx,
-                     int          ny,
-                     float        kx,
-                     float        ky,
-                     uint32_t     seed,
-                     int          octaves,
-                     float        weight,
-                     float        persistence,
-                     float        lacunarity,
-                     const float *ctrl,
-                     const float *noise_x,
-                     const float *noise_y,
-                     const float *bbox4,
-                     const int   *period2,
-                     hipStream_t  st)
+template <int TYPE>
+__global__ void __launch_bounds__(256) k_noise(float *__restrict__ out,
+                                               const float *__restrict__ nxp,
+                                               const float *__restrict__ nyp,
+                                               NoiseArgs a)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  size_t idx = (size_t)j * a.nx + i;
+  float  x, y;
+  noise_xy(i, j, a, nxp ? nxp[idx] : 0.f, nyp ? nyp[idx] : 0.f, x, y);
+  out[idx] = base_noise<TYPE>(x, y, a.fseed, a.per_x, a.per_y);
+}
+
+static NoiseArgs make_noise_args(int          nx,
+                                 int          ny,
+                                 float        kx,
+                                 float        ky,
+                                 uint32_t     seed,
+                                 int          octaves,
+                                 float        weight,
+                                 float        persistence,
+                                 float        lacunarity,
+                                 const float *bbox4,
+                                 const int   *period2)
 {
   NoiseArgs a;
   a.nx = nx;
@@ -142,10 +289,79 @@ int launch_noise_fbm(float       *out,
   a.b3 = bbox4 ? bbox4[3] : 1.f;
   a.per_x = period2 ? period2[0] : 0;
   a.per_y = period2 ? period2[1] : 0;
+  return a;
+}
+
+int launch_noise_fbm(float       *out,
+                     int          nx,
+                     int          ny,
+                     int          noise_type,
+                     float        kx,
+                     float        ky,
+                     uint32_t     seed,
+                     int          octaves,
+                     float        weight,
+                     float        persistence,
+                     float        lacunarity,
+                     const float *ctrl,
+                     const float *noise_x,
+                     const float *noise_y,
+                     const float *bbox4,
+                     const int   *period2,
+                     hipStream_t  st)
+{
+  const NoiseArgs a = make_noise_args(nx, ny, kx, ky, seed, octaves, weight, persistence,
+                                      lacunarity, bbox4, period2);
   dim3 blk(64, 4);
   dim3 grd((nx + 63) / 64, (ny + 3) / 4);
-  k_noise_fbm_perlin<<<grd, blk, 0, st>>>(out, ctrl, noise_x, noise_y, a);
+#define HMG_FBM(T) k_noise_fbm<T><<<grd, blk, 0, st>>>(out, ctrl, noise_x, noise_y, a)
+  switch (noise_type)
+  {
+  case 1: HMG_FBM(1); break;
+  case 2: HMG_FBM(2); break;
+  case 3: HMG_FBM(3); break;
+  case 4: HMG_FBM(4); break;
+  case 6: HMG_FBM(6); break;
+  case 7: HMG_FBM(7); break;
+  case 9: HMG_FBM(9); break;
+  case 10: HMG_FBM(10); break;
+  default: HMG_FBM(0); break; // no kernel branch: zeros
+  }
+#undef HMG_FBM
   return check_launch("noise_fbm");
+}
+
+int launch_noise(float       *out,
+                 int          nx,
+                 int          ny,
+                 int          noise_type,
+                 float        kx,
+                 float        ky,
+                 uint32_t     seed,
+                 const float *noise_x,
+                 const float *noise_y,
+                 const float *bbox4,
+                 const int   *period2,
+                 hipStream_t  st)
+{
+  const NoiseArgs a = make_noise_args(nx, ny, kx, ky, seed, 1, 0.f, 0.f, 0.f, bbox4, period2);
+  dim3 blk(64, 4);
+  dim3 grd((nx + 63) / 64, (ny + 3) / 4);
+#define HMG_NOISE(T) k_noise<T><<<grd, blk, 0, st>>>(out, noise_x, noise_y, a)
+  switch (noise_type)
+  {
+  case 1: HMG_NOISE(1); break;
+  case 2: HMG_NOISE(2); break;
+  case 3: HMG_NOISE(3); break;
+  case 4: HMG_NOISE(4); break;
+  case 6: HMG_NOISE(6); break;
+  case 7: HMG_NOISE(7); break;
+  case 9: HMG_NOISE(9); break;
+  case 10: HMG_NOISE(10); break;
+  default: HMG_NOISE(0); break;
+  }
+#undef HMG_NOISE
+  return check_launch("noise");
 }
 
 static int check_noise_args(int nx, int ny, int noise_type, int octaves)
@@ -154,9 +370,9 @@ static int check_noise_args(int nx, int ny, int noise_type, int octaves)
   if ((size_t)nx * (size_t)ny > ((size_t)1 << 31))
     return set_error(HMG_ERR_INVALID, "grid larger than 2^31 cells");
   if (octaves < 0) return set_error(HMG_ERR_INVALID, "octaves must be >= 0");
-  if (noise_type != 1)
-    return set_error(HMG_ERR_UNSUPPORTED,
-                     "noise type %d not implemented (only PERLIN = 1)", noise_type);
+  // hmap::NoiseType has 13 values (PARBERRY 0 .. WORLEY_VALUE 12)
+  if (noise_type < 0 || noise_type > 12)
+    return set_error(HMG_ERR_INVALID, "noise type %d is not a hmap::NoiseType", noise_type);
   return HMG_OK;
 }
 
@@ -272,7 +488,7 @@ int hmg_noise_fbm_dev(hmg_ctx     *ctx,
   if (!out_dev) return set_error(HMG_ERR_INVALID, "null output");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   HMG_TRY(ctx_enter(ctx));
-  return launch_noise_fbm(out_dev, nx, ny, kx, ky, seed, octaves, weight, persistence,
+  return launch_noise_fbm(out_dev, nx, ny, noise_type, kx, ky, seed, octaves, weight, persistence,
                           lacunarity, ctrl_dev, noise_x_dev, noise_y_dev, bbox4, period2,
                           ctx->stream);
 }
@@ -322,9 +538,74 @@ int hmg_noise_fbm(hmg_ctx     *ctx,
     HMG_HIP(hipMemcpyAsync(gy.ptr, noise_y_host, bytes, hipMemcpyHostToDevice, ctx->stream));
     py = gy.as<float>();
   }
-  HMG_TRY(launch_noise_fbm(out.as<float>(), nx, ny, kx, ky, seed, octaves, weight,
+  HMG_TRY(launch_noise_fbm(out.as<float>(), nx, ny, noise_type, kx, ky, seed, octaves, weight,
                            persistence, lacunarity, pc, px, py, bbox4, period2,
                            ctx->stream));
+  HMG_HIP(hipMemcpyAsync(out_host, out.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HMG_HIP(hipStreamSynchronize(ctx->stream));
+  return HMG_OK;
+}
+
+int hmg_noise_dev(hmg_ctx     *ctx,
+                  float       *out_dev,
+                  int          nx,
+                  int          ny,
+                  int          noise_type,
+                  float        kx,
+                  float        ky,
+                  uint32_t     seed,
+                  const float *noise_x_dev,
+                  const float *noise_y_dev,
+                  const float *bbox4,
+                  const int   *period2)
+{
+  clear_error();
+  HMG_TRY(check_noise_args(nx, ny, noise_type, 0));
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!out_dev) return set_error(HMG_ERR_INVALID, "null output");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  return launch_noise(out_dev, nx, ny, noise_type, kx, ky, seed, noise_x_dev, noise_y_dev, bbox4,
+                      period2, ctx->stream);
+}
+
+int hmg_noise(hmg_ctx     *ctx,
+              float       *out_host,
+              int          nx,
+              int          ny,
+              int          noise_type,
+              float        kx,
+              float        ky,
+              uint32_t     seed,
+              const float *noise_x_host,
+              const float *noise_y_host,
+              const float *bbox4,
+              const int   *period2)
+{
+  clear_error();
+  HMG_TRY(check_noise_args(nx, ny, noise_type, 0));
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!out_host) return set_error(HMG_ERR_INVALID, "null output");
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  size_t bytes = (size_t)nx * ny * sizeof(float);
+  DevBuf out, gx, gy;
+  HMG_TRY(out.get(ctx->pool, bytes));
+  const float *px = nullptr, *py = nullptr;
+  if (noise_x_host)
+  {
+    HMG_TRY(gx.get(ctx->pool, bytes));
+    HMG_HIP(hipMemcpyAsync(gx.ptr, noise_x_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    px = gx.as<float>();
+  }
+  if (noise_y_host)
+  {
+    HMG_TRY(gy.get(ctx->pool, bytes));
+    HMG_HIP(hipMemcpyAsync(gy.ptr, noise_y_host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    py = gy.as<float>();
+  }
+  HMG_TRY(launch_noise(out.as<float>(), nx, ny, noise_type, kx, ky, seed, px, py, bbox4,
+                       period2, ctx->stream));
   HMG_HIP(hipMemcpyAsync(out_host, out.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
   HMG_HIP(hipStreamSynchronize(ctx->stream));
   return HMG_OK;

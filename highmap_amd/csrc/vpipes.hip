@@ -37,6 +37,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "hmg_internal.hpp"
 
@@ -926,6 +929,132 @@ static int dispatch_step(hmg_session *s)
   }
 }
 
+// ------------------------------------------------ online shape tuning
+//
+// The fused kernel's speed depends on rows per wavefront and load-ahead in
+// steps that follow how the wavefronts fill the CUs, which no formula
+// predicts well.  A session of at least 2^20 cells in fused mode therefore
+// runs its first iterations as probes: each launches one candidate shape
+// between two events (two rounds over the candidates, after one warm-up
+// launch); at the first iteration after the probes the events are read,
+// the shape with the least time (minimum over its two rounds) is kept and
+// cached per (storage, nx, ny, rain plane, k_discharge_exp == 1) for later
+// sessions of the process.  The probes are ordinary iterations.
+
+namespace
+{
+struct TuneKey
+{
+  int  storage, nx, ny, rain, ke1;
+  bool operator<(const TuneKey &o) const
+  {
+    return std::tie(storage, nx, ny, rain, ke1) < std::tie(o.storage, o.nx, o.ny, o.rain, o.ke1);
+  }
+};
+std::mutex                                   g_shape_mu;
+std::map<TuneKey, std::pair<int, int>>       g_shape_cache;
+constexpr int                                TUNE_CAND[][2] = {{24, 1}, {32, 1}, {40, 1}, {47, 1},
+                                                               {56, 1}, {64, 1}, {47, 0}, {56, 0}};
+constexpr int                                N_CAND = sizeof(TUNE_CAND) / sizeof(TUNE_CAND[0]);
+constexpr int                                N_PROBE = 1 + 2 * N_CAND; // warm-up + 2 rounds
+
+TuneKey tune_key(const hmg_session *s)
+{
+  return TuneKey{s->storage, s->nx, s->ny, s->rain.ptr ? 1 : 0, s->k.kexp == 1.f ? 1 : 0};
+}
+} // namespace
+
+static void tune_init(hmg_session *s)
+{
+  s->tune_pos = -1;
+  if (s->p.mode != HMG_VP_FUSED || (size_t)s->nx * s->ny < ((size_t)1 << 20)) return;
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto                        it = g_shape_cache.find(tune_key(s));
+    if (it != g_shape_cache.end())
+    {
+      s->chunk_rows = it->second.first;
+      s->prefetch = it->second.second;
+      return;
+    }
+  }
+  s->tune_ev.assign(2 * N_PROBE, nullptr);
+  for (hipEvent_t &e : s->tune_ev)
+    if (hipEventCreate(&e) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      for (hipEvent_t &f : s->tune_ev)
+        if (f) (void)hipEventDestroy(f);
+      s->tune_ev.clear();
+      return; // no tuning: keep the default shape
+    }
+  s->tune_pos = 0;
+}
+
+static void tune_stop(hmg_session *s)
+{
+  for (hipEvent_t e : s->tune_ev) (void)hipEventDestroy(e);
+  s->tune_ev.clear();
+  s->tune_pos = -1;
+}
+
+// Picks the best probed shape (blocks until the probes have run).
+static void tune_finish(hmg_session *s)
+{
+  float best = 0.f;
+  int   bi = -1;
+  if (hipEventSynchronize(s->tune_ev[2 * N_PROBE - 1]) == hipSuccess)
+    for (int c = 0; c < N_CAND; c++)
+    {
+      float t = -1.f;
+      for (int r = 0; r < 2; r++)
+      {
+        const int q = 1 + r * N_CAND + c;
+        float     ms = 0.f;
+        if (hipEventElapsedTime(&ms, s->tune_ev[2 * q], s->tune_ev[2 * q + 1]) == hipSuccess)
+          t = t < 0.f ? ms : std::min(t, ms);
+      }
+      if (t >= 0.f && (bi < 0 || t < best))
+      {
+        best = t;
+        bi = c;
+      }
+    }
+  (void)hipGetLastError();
+  if (bi >= 0)
+  {
+    s->chunk_rows = TUNE_CAND[bi][0];
+    s->prefetch = TUNE_CAND[bi][1];
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shape_cache[tune_key(s)] = {s->chunk_rows, s->prefetch};
+  }
+  tune_stop(s);
+}
+
+static int dispatch_step(hmg_session *s);
+
+// One hydraulic iteration, as a probe while the session is tuning.
+static int tuned_step(hmg_session *s)
+{
+  if (s->tune_pos < 0) return dispatch_step(s);
+  if (s->tune_pos >= N_PROBE)
+  {
+    tune_finish(s);
+    return dispatch_step(s);
+  }
+  const int q = s->tune_pos++;
+  if (q > 0)
+  {
+    const int c = (q - 1) % N_CAND;
+    s->chunk_rows = TUNE_CAND[c][0];
+    s->prefetch = TUNE_CAND[c][1];
+  }
+  HMG_HIP(hipEventRecord(s->tune_ev[2 * q], s->ctx->stream));
+  HMG_TRY(dispatch_step(s));
+  HMG_HIP(hipEventRecord(s->tune_ev[2 * q + 1], s->ctx->stream));
+  return HMG_OK;
+}
+
 static int session_create_common(hmg_ctx                 *ctx,
                                  const float             *z,
                                  bool                     z_on_dev,
@@ -1021,6 +1150,7 @@ static int session_create_common(hmg_ctx                 *ctx,
     delete s;
     return rc;
   }
+  tune_init(s);
   *out = s;
   return HMG_OK;
 }
@@ -1067,7 +1197,7 @@ int hmg_session_vpipes(hmg_session *s, int iterations)
   if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
   std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
   HMG_TRY(ctx_enter(s->ctx));
-  for (int it = 0; it < iterations; it++) HMG_TRY(dispatch_step(s));
+  for (int it = 0; it < iterations; it++) HMG_TRY(tuned_step(s));
   return HMG_OK;
 }
 
@@ -1178,8 +1308,20 @@ int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch)
     return set_error(HMG_ERR_INVALID, "chunk_rows must be in [1, 65536]");
   if (prefetch != 0 && prefetch != 1) return set_error(HMG_ERR_INVALID, "prefetch must be 0 or 1");
   std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  tune_stop(s); // the caller's shape wins over online tuning
   s->chunk_rows = chunk_rows;
   s->prefetch = prefetch;
+  return HMG_OK;
+}
+
+int hmg_session_get_tuning(hmg_session *s, int *chunk_rows, int *prefetch, int *tuning)
+{
+  clear_error();
+  if (!s) return set_error(HMG_ERR_INVALID, "null session");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  if (chunk_rows) *chunk_rows = s->chunk_rows;
+  if (prefetch) *prefetch = s->prefetch;
+  if (tuning) *tuning = s->tune_pos >= 0 ? 1 : 0;
   return HMG_OK;
 }
 

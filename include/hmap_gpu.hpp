@@ -2,6 +2,7 @@
 //
 // Declares, with the reference's names, signatures and default arguments:
 //   hmap::gpu::init_opencl        HighMap/include/highmap/opencl/gpu_opencl.hpp:30
+//   hmap::gpu::noise              HighMap/include/highmap/primitives/coherent_noise.hpp:804-812
 //   hmap::gpu::noise_fbm          HighMap/include/highmap/primitives/coherent_noise.hpp:821-834
 //   hmap::gpu::thermal (3 overloads)  HighMap/include/highmap/erosion.hpp:1784-1801
 //   hmap::gpu::thermal_auto_bedrock (3) HighMap/include/highmap/erosion.hpp:1824-1838
@@ -95,6 +96,16 @@ namespace gpu
 {
 
 bool init_opencl();
+
+Array noise(NoiseType     noise_type,
+            glm::ivec2    shape,
+            glm::vec2     kw,
+            std::uint32_t seed,
+            const Array  *p_noise_x = nullptr,
+            const Array  *p_noise_y = nullptr,
+            const Array  *p_stretching = nullptr,
+            glm::vec4     bbox = {0.f, 1.f, 0.f, 1.f},
+            glm::ivec2    period = {0, 0});
 
 Array noise_fbm(NoiseType     noise_type,
                 glm::ivec2    shape,

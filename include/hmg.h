@@ -1,7 +1,8 @@
 /*
  * hmg.h -- C ABI of libhmg, the MI355X (gfx950) implementation of HighMap's
- * grid-erosion hot path: gpu::noise_fbm (PERLIN), gpu::thermal and
- * gpu::hydraulic_vpipes.
+ * grid-erosion hot path: gpu::noise_fbm / gpu::noise, gpu::thermal (and
+ * thermal_auto_bedrock, sediment_deposition), gpu::hydraulic_vpipes and
+ * gpu::flow_simulation.
  *
  * This header replaces the OpenCL runtime layer the reference binds through
  * (HighMap/src/gpu_opencl/gpu_opencl.cpp and the CLWrapper `Run` calls in
@@ -120,9 +121,14 @@ void *hmg_stream(hmg_ctx *ctx);
 /* hmap::gpu::noise_fbm (HighMap/src/primitives/coherent_noise/
  * primitives_gpu.cpp:543-591; defaults coherent_noise.hpp:821-834:
  * octaves 8, weight 0.7, persistence 0.5, lacunarity 2, bbox {0,1,0,1},
- * period {0,0}).  noise_type follows hmap::NoiseType; only PERLIN (1) is
- * implemented, other types return HMG_ERR_UNSUPPORTED.  ctrl, noise_x,
- * noise_y may be NULL; bbox and period may be NULL for the defaults. */
+ * period {0,0}).  noise_type follows hmap::NoiseType
+ * (HighMap/include/highmap/functions.hpp:66-81): PERLIN 1, PERLIN_BILLOW 2,
+ * PERLIN_HALF 3, SIMPLEX2 4, VALUE 6, VALUE_CUBIC 7, VALUE_LINEAR 9 and
+ * WORLEY 10 are computed (noise.cl:28-383); the reference kernel writes 0 for
+ * PARBERRY 0, SIMPLEX2S 5, VALUE_DELAUNAY 8, WORLEY_DOUBLE 11 and
+ * WORLEY_VALUE 12, and so does this call; other values are
+ * HMG_ERR_INVALID.  ctrl, noise_x, noise_y may be NULL; bbox and period may
+ * be NULL for the defaults. */
 int hmg_noise_fbm(hmg_ctx     *ctx,
                   float       *out_host,
                   int          nx,
@@ -158,6 +164,35 @@ int hmg_noise_fbm_dev(hmg_ctx     *ctx,
                       const float *noise_y_dev,
                       const float *bbox4,
                       const int   *period2);
+
+/* hmap::gpu::noise, one octave (HighMap/src/primitives/coherent_noise/
+ * primitives_gpu.cpp:504-541, kernel noise.cl:385-454; defaults
+ * coherent_noise.hpp:804-812: bbox {0,1,0,1}, period {0,0}); types as for
+ * hmg_noise_fbm. */
+int hmg_noise(hmg_ctx     *ctx,
+              float       *out_host,
+              int          nx,
+              int          ny,
+              int          noise_type,
+              float        kx,
+              float        ky,
+              uint32_t     seed,
+              const float *noise_x_host,
+              const float *noise_y_host,
+              const float *bbox4,
+              const int   *period2);
+int hmg_noise_dev(hmg_ctx     *ctx,
+                  float       *out_dev,
+                  int          nx,
+                  int          ny,
+                  int          noise_type,
+                  float        kx,
+                  float        ky,
+                  uint32_t     seed,
+                  const float *noise_x_dev,
+                  const float *noise_y_dev,
+                  const float *bbox4,
+                  const int   *period2);
 
 /* hmap::remap(array, vmin, vmax) (HighMap/src/range/range.cpp:342-361) on a
  * device buffer; min and max are reduced on the device. */
@@ -337,9 +372,15 @@ int hmg_session_export(hmg_session *s,
 /* Waits, then returns sum(z), sum(s), sum(d) in FP64 (mass check). */
 int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d);
 /* Launch shape of the fused kernel: rows each wavefront streams through
- * (default 64) and whether each row's loads are issued one row ahead
- * (default 1).  Results do not depend on either. */
+ * and whether each row's loads are issued one row ahead.  Results do not
+ * depend on either.  By default a fused session of at least 2^20 cells
+ * tunes the shape online: its first 17 hydraulic iterations each run one
+ * candidate shape between two events, the next one keeps the fastest (a
+ * process-wide cache per storage, shape, rain plane and exponent skips the
+ * probes for later sessions).  Setting the shape ends the tuning. */
 int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch);
+/* The current shape; *tuning is 1 while the online probes still run. */
+int hmg_session_get_tuning(hmg_session *s, int *chunk_rows, int *prefetch, int *tuning);
 /* Device pointer of the session's current z plane (FP32, nx*ny). */
 float *hmg_session_z_dev(hmg_session *s);
 int    hmg_session_sync(hmg_session *s);

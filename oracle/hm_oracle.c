@@ -200,23 +200,171 @@ float orc_base_perlin(float px, float py, float fseed, int per_x, int per_y)
   return 1.42857f * orc_lerp(nx0, nx1, uy);
 }
 
-/* base_perlin_fbm (noise.cl:194-216).
- * Parity pinned: octaves=1 gives 0.6*P(pos); weight=0 gives amplitudes
- * 0.6*pers^k (closed forms in tests/test_oracle_noise.py). */
-static float orc_perlin_fbm(float px,
-                            float py,
-                            int   octaves,
-                            float weight,
-                            float persistence,
-                            float lacunarity,
-                            float fseed,
-                            int   per_x,
-                            int   per_y)
+/* hash22f (_common_rand.cl:28-34): two hashes of the lattice point, with
+ * the weight pairs (127.1, 311.7) and (269.5, 183.3). */
+static void orc_hash22f(float px, float py, float fseed, float *hx, float *hy)
+{
+  float qx = px * 127.1f + py * 311.7f;
+  float qy = px * 269.5f + py * 183.3f;
+  *hx = orc_fractf(orc_sinf(qx + fseed) * 43758.5453123f);
+  *hy = orc_fractf(orc_sinf(qy + fseed) * 43758.5453123f);
+}
+
+/* base_simplex2 (noise.cl:65-99): 2-D simplex noise on the skewed lattice
+ * with K1 = (sqrt(3)-1)/2 and K2 = (3-sqrt(3))/6 as float literals, the
+ * input scaled by 0.5, contributions (0.5 - |d|^2)^4 (g . d), scale 100.
+ * No period. */
+static float orc_base_simplex2(float px, float py, float fseed)
+{
+  const float K1 = 0.366025403f, K2 = 0.211324865f;
+  float       x = 0.5f * px, y = 0.5f * py;
+  float       s = x * K1 + y * K1;
+  float       ix = floorf(x + s), iy = floorf(y + s);
+  float       t = ix * K2 + iy * K2;
+  float       ax = (x - ix) + t, ay = (y - iy) + t;
+  float       m = ax < ay ? 0.f : 1.f; /* step(a.y, a.x) */
+  float       ox = m, oy = 1.f - m;
+  float       bx = (ax - ox) + K2, by = (ay - oy) + K2;
+  float       cx = (ax - 1.f) + 2.f * K2, cy = (ay - 1.f) + 2.f * K2;
+  float       n[3];
+  const float vx[3] = {ax, bx, cx}, vy[3] = {ay, by, cy};
+  const float lx[3] = {ix, ix + ox, ix + 1.f}, ly[3] = {iy, iy + oy, iy + 1.f};
+  for (int k = 0; k < 3; k++)
+  {
+    float w = fmaxf(0.5f - (vx[k] * vx[k] + vy[k] * vy[k]), 0.f);
+    /* gradient: grad22f at the vertex, dotted with the offset */
+    n[k] = w * w * w * w * orc_corner(lx[k], ly[k], fseed, vx[k], vy[k]);
+  }
+  return 100.f * (n[0] + n[1] + n[2]);
+}
+
+/* base_value (noise.cl:101-119) and base_value_linear (:150-166): bilinear
+ * blend of lattice hashes, weights smoothstep3 (x*x*(3 - 2x)) or linear,
+ * mapped to [-1, 1]. */
+static float orc_base_value(float px, float py, float fseed, int per_x, int per_y, int cubic_fade)
+{
+  float ix = floorf(px), iy = floorf(py);
+  float fx = orc_fractf(px), fy = orc_fractf(py);
+  float x0 = orc_wrap(ix, per_x), x1 = orc_wrap(ix + 1.f, per_x);
+  float y0 = orc_wrap(iy, per_y), y1 = orc_wrap(iy + 1.f, per_y);
+  float v00 = orc_hash12f(x0, y0, fseed), v10 = orc_hash12f(x1, y0, fseed);
+  float v01 = orc_hash12f(x0, y1, fseed), v11 = orc_hash12f(x1, y1, fseed);
+  float ux = fx, uy = fy;
+  if (cubic_fade)
+  {
+    ux = fx * fx * (3.f - 2.f * fx);
+    uy = fy * fy * (3.f - 2.f * fy);
+  }
+  float nx0 = orc_lerp(v00, v10, ux);
+  float nx1 = orc_lerp(v01, v11, ux);
+  return 2.f * orc_lerp(nx0, nx1, uy) - 1.f;
+}
+
+/* cubic_interp (_common_math.cl:30-37), evaluated as written:
+ * a t^3 + b t^2 + c t + d with a = p3 - p2 - p0 + p1, b = p0 - p1 - a,
+ * c = p2 - p0, d = p1 (left to right). */
+static float orc_cubic(float p0, float p1, float p2, float p3, float t)
+{
+  float a = p3 - p2 - p0 + p1;
+  float b = p0 - p1 - a;
+  float c = p2 - p0;
+  return a * t * t * t + b * t * t + c * t + p1;
+}
+
+/* base_value_cubic (noise.cl:121-148): 4x4 lattice hashes, cubic rows then
+ * the column, scaled 1.43 * (v - 0.5). */
+static float orc_base_value_cubic(float px, float py, float fseed, int per_x, int per_y)
+{
+  float ix = floorf(px), iy = floorf(py);
+  float fx = orc_fractf(px), fy = orc_fractf(py);
+  float row[4];
+  for (int dy = -1; dy <= 2; dy++)
+  {
+    float v[4];
+    for (int dx = -1; dx <= 2; dx++)
+      v[dx + 1] = orc_hash12f(orc_wrap(ix + (float)dx, per_x), orc_wrap(iy + (float)dy, per_y),
+                              fseed);
+    row[dy + 1] = orc_cubic(v[0], v[1], v[2], v[3], fx);
+  }
+  return 1.43f * (orc_cubic(row[0], row[1], row[2], row[3], fy) - 0.5f);
+}
+
+/* base_worley (noise.cl:168-190): squared distance to the nearest feature
+ * point (one per lattice cell, hash22f offset) over the 3x3 cells around,
+ * scanned dx outer, dy inner; 1.66 * d^2 - 1. */
+static float orc_base_worley(float px, float py, float fseed, int per_x, int per_y)
+{
+  float ix = floorf(px), iy = floorf(py);
+  float fx = orc_fractf(px), fy = orc_fractf(py);
+  float dmin = FLT_MAX;
+  for (int dx = -1; dx <= 1; dx++)
+    for (int dy = -1; dy <= 1; dy++)
+    {
+      float hx, hy;
+      orc_hash22f(orc_wrap(ix + (float)dx, per_x), orc_wrap(iy + (float)dy, per_y), fseed, &hx,
+                  &hy);
+      float ex = fx - ((float)dx + hx), ey = fy - ((float)dy + hy);
+      dmin = fminf(dmin, ex * ex + ey * ey);
+    }
+  return 1.66f * dmin - 1.f;
+}
+
+/* max_smooth(a, b, k) (_common_math.cl:52-56) with pow_int(h, 3)
+ * (_common_math.cl:10-18): ((1*h)*h)*h. */
+static float orc_max_smooth(float a, float b, float k)
+{
+  float h = fmaxf(k - fabsf(a - b), 0.f) / k;
+  float h3 = 1.f;
+  for (int q = 0; q < 3; q++) h3 *= h;
+  return fmaxf(a, b) + h3 * k / 6.f;
+}
+
+/* The single-octave value of each hmap::NoiseType the kernels implement
+ * (noise.cl:385-454): PERLIN 1, PERLIN_BILLOW 2 (2|P| - 1), PERLIN_HALF 3
+ * (max_smooth(P, 0, 0.5)), SIMPLEX2 4, VALUE 6, VALUE_CUBIC 7, VALUE_LINEAR
+ * 9, WORLEY 10.  Other types are 0, as the kernels write. */
+float orc_base_noise(int type, float px, float py, float fseed, int per_x, int per_y)
+{
+  switch (type)
+  {
+  case 1: return orc_base_perlin(px, py, fseed, per_x, per_y);
+  case 2: return 2.f * fabsf(orc_base_perlin(px, py, fseed, per_x, per_y)) - 1.f;
+  case 3: return orc_max_smooth(orc_base_perlin(px, py, fseed, per_x, per_y), 0.f, 0.5f);
+  case 4: return orc_base_simplex2(px, py, fseed);
+  case 6: return orc_base_value(px, py, fseed, per_x, per_y, 1);
+  case 7: return orc_base_value_cubic(px, py, fseed, per_x, per_y);
+  case 9: return orc_base_value(px, py, fseed, per_x, per_y, 0);
+  case 10: return orc_base_worley(px, py, fseed, per_x, per_y);
+  default: return 0.f;
+  }
+}
+
+int orc_noise_type_supported(int type)
+{
+  return type == 1 || type == 2 || type == 3 || type == 4 || type == 6 || type == 7 ||
+         type == 9 || type == 10;
+}
+
+/* The fBm of each type (noise.cl:194-383): amplitude 0.6, feedback
+ * na *= (1 - w) + w * min(v + 1, 2) * 0.5, then na *= persistence; the
+ * frequency grows by lacunarity and the period by scale_period (simplex has
+ * none).  Parity pinned: octaves=1 gives 0.6 * base; weight=0 gives
+ * amplitudes 0.6 * pers^k (closed forms in tests/test_oracle_noise.py). */
+static float orc_fbm(int   type,
+                     float px,
+                     float py,
+                     int   octaves,
+                     float weight,
+                     float persistence,
+                     float lacunarity,
+                     float fseed,
+                     int   per_x,
+                     int   per_y)
 {
   float n = 0.f, nf = 1.f, na = 0.6f;
   for (int k = 0; k < octaves; k++)
   {
-    float v = orc_base_perlin(px * nf, py * nf, fseed, per_x, per_y);
+    float v = orc_base_noise(type, px * nf, py * nf, fseed, per_x, per_y);
     n += v * na;
     na *= (1.f - weight) + weight * fminf(v + 1.f, 2.f) * 0.5f;
     na *= persistence;
@@ -227,9 +375,28 @@ static float orc_perlin_fbm(float px,
   return n;
 }
 
-/* noise_fbm kernel, PERLIN branch (noise.cl:456-504) with g_to_xy
- * (_common_index.cl:8-24); host defaults from
- * HighMap/include/highmap/primitives/coherent_noise.hpp:821-834. */
+/* g_to_xy (_common_index.cl:8-24). */
+static void orc_g_to_xy(int          i,
+                        int          j,
+                        int          nx,
+                        int          ny,
+                        float        kx,
+                        float        ky,
+                        float        dx,
+                        float        dy,
+                        const float *bbox4,
+                        float       *x,
+                        float       *y)
+{
+  float u = (float)i / (float)nx;
+  float w = (float)j / (float)ny;
+  *x = kx * (u * (bbox4[1] - bbox4[0]) + bbox4[0]) + kx * dx;
+  *y = ky * (w * (bbox4[3] - bbox4[2]) + bbox4[2]) + ky * dy;
+}
+
+/* noise_fbm kernel (noise.cl:456-577) with the host defaults of
+ * HighMap/include/highmap/primitives/coherent_noise.hpp:821-834.  Types the
+ * kernel does not implement give 0 everywhere (returns 1 then). */
 int orc_noise_fbm_f32(float       *out,
                       int          nx,
                       int          ny,
@@ -247,13 +414,15 @@ int orc_noise_fbm_f32(float       *out,
                       const float *bbox4,
                       const int   *period2)
 {
-  if (noise_type != 1) return -1;
   const float bbox_def[4] = {0.f, 1.f, 0.f, 1.f};
   const int   per_def[2] = {0, 0};
   if (!bbox4) bbox4 = bbox_def;
   if (!period2) period2 = per_def;
 
   const float fseed = orc_seed_to_fseed(seed);
+  /* simplex ignores the period (noise.cl:266-285) */
+  const int px = noise_type == 4 ? 0 : period2[0];
+  const int py = noise_type == 4 ? 0 : period2[1];
 
 #pragma omp parallel for schedule(static)
   for (int j = 0; j < ny; j++)
@@ -264,23 +433,44 @@ int orc_noise_fbm_f32(float       *out,
       float  dx = noise_x ? noise_x[idx] : 0.f;
       float  dy = noise_y ? noise_y[idx] : 0.f;
       float  w = (1.f - ct) + ct * weight;
-
-      float x = (float)i / (float)nx;
-      float y = (float)j / (float)ny;
-      x = kx * (x * (bbox4[1] - bbox4[0]) + bbox4[0]) + kx * dx;
-      y = ky * (y * (bbox4[3] - bbox4[2]) + bbox4[2]) + ky * dy;
-
-      out[idx] = orc_perlin_fbm(x,
-                                y,
-                                octaves,
-                                w,
-                                persistence,
-                                lacunarity,
-                                fseed,
-                                period2[0],
-                                period2[1]);
+      float  x, y;
+      orc_g_to_xy(i, j, nx, ny, kx, ky, dx, dy, bbox4, &x, &y);
+      out[idx] = orc_fbm(noise_type, x, y, octaves, w, persistence, lacunarity, fseed, px, py);
     }
-  return 0;
+  return orc_noise_type_supported(noise_type) ? 0 : 1;
+}
+
+/* noise kernel (noise.cl:385-454), the single-octave hmap::gpu::noise
+ * (HighMap/src/primitives/coherent_noise/primitives_gpu.cpp:504-541). */
+int orc_noise_f32(float       *out,
+                  int          nx,
+                  int          ny,
+                  int          noise_type,
+                  float        kx,
+                  float        ky,
+                  uint32_t     seed,
+                  const float *noise_x,
+                  const float *noise_y,
+                  const float *bbox4,
+                  const int   *period2)
+{
+  const float bbox_def[4] = {0.f, 1.f, 0.f, 1.f};
+  const int   per_def[2] = {0, 0};
+  if (!bbox4) bbox4 = bbox_def;
+  if (!period2) period2 = per_def;
+  const float fseed = orc_seed_to_fseed(seed);
+
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++)
+    {
+      size_t idx = (size_t)j * nx + i;
+      float  x, y;
+      orc_g_to_xy(i, j, nx, ny, kx, ky, noise_x ? noise_x[idx] : 0.f,
+                  noise_y ? noise_y[idx] : 0.f, bbox4, &x, &y);
+      out[idx] = orc_base_noise(noise_type, x, y, fseed, period2[0], period2[1]);
+    }
+  return orc_noise_type_supported(noise_type) ? 0 : 1;
 }
 
 /* remap to [vmin, vmax] (HighMap/src/range/range.cpp:342-361). */

@@ -77,8 +77,18 @@ float    orc_cosf(float x);
 float    orc_fractf(float x);
 float    orc_hash12f(float px, float py, float fseed);
 float    orc_base_perlin(float px, float py, float fseed, int per_x, int per_y);
+/* Single-octave value of hmap::NoiseType `type` (0 for types the kernels do
+ * not implement), and whether the kernels implement it. */
+float    orc_base_noise(int type, float px, float py, float fseed, int per_x, int per_y);
+int      orc_noise_type_supported(int type);
+/* gpu::noise (single octave); returns 1 (zeros written) for types the
+ * kernels do not implement. */
+int orc_noise_f32(float *out, int nx, int ny, int noise_type, float kx, float ky, uint32_t seed,
+                  const float *noise_x, const float *noise_y, const float *bbox4,
+                  const int *period2);
 
-/* --- noise_fbm (PERLIN branch only); returns 0, or -1 for other types --- */
+/* --- noise_fbm, every implemented type; returns 0, or 1 (zeros written)
+ * for types the kernels do not implement --- */
 int orc_noise_fbm_f32(float       *out,
                       int          nx,
                       int          ny,

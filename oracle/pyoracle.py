@@ -95,6 +95,13 @@ def lib():
     L.orc_hash12f.restype = C.c_float
     L.orc_base_perlin.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
     L.orc_base_perlin.restype = C.c_float
+    L.orc_base_noise.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
+    L.orc_base_noise.restype = C.c_float
+    L.orc_noise_type_supported.argtypes = [C.c_int]
+    L.orc_noise_type_supported.restype = C.c_int
+    L.orc_noise_f32.argtypes = [F32P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, u32,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_noise_f32.restype = C.c_int
     L.orc_noise_fbm_f32.argtypes = [F32P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                     u32, C.c_int, C.c_float, C.c_float, C.c_float,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -152,7 +159,8 @@ def _ptr(a):
 def noise_fbm(nx, ny, kw=(4.0, 4.0), seed=0, octaves=8, weight=0.7, persistence=0.5,
               lacunarity=2.0, ctrl=None, noise_x=None, noise_y=None,
               bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0), noise_type=1):
-    """FP32 restatement of hmap::gpu::noise_fbm (PERLIN)."""
+    """FP32 restatement of hmap::gpu::noise_fbm (every hmap::NoiseType the
+    kernels implement; others are all zeros)."""
     out = np.zeros((ny, nx), np.float32)
     keep = []
 
@@ -169,9 +177,27 @@ def noise_fbm(nx, ny, kw=(4.0, 4.0), seed=0, octaves=8, weight=0.7, persistence=
     rc = lib().orc_noise_fbm_f32(out, nx, ny, noise_type, kw[0], kw[1], seed & 0xFFFFFFFF,
                                  octaves, weight, persistence, lacunarity,
                                  opt(ctrl), opt(noise_x), opt(noise_y), _ptr(bb), _ptr(pe))
-    if rc != 0:
-        raise ValueError("unsupported noise type %d" % noise_type)
+    if rc not in (0, 1):
+        raise ValueError("noise_fbm failed (%d)" % rc)
     return out
+
+
+def noise(nx, ny, kw=(4.0, 4.0), seed=0, noise_type=1, noise_x=None, noise_y=None,
+          bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)):
+    """FP32 restatement of hmap::gpu::noise (single octave)."""
+    out = np.zeros((ny, nx), np.float32)
+    keep = [np.ascontiguousarray(a, np.float32) if a is not None else None
+            for a in (noise_x, noise_y)]
+    bb = np.asarray(bbox, np.float32)
+    pe = np.asarray(period, np.int32)
+    lib().orc_noise_f32(out, nx, ny, noise_type, kw[0], kw[1], seed & 0xFFFFFFFF,
+                        _ptr(keep[0]), _ptr(keep[1]), _ptr(bb), _ptr(pe))
+    return out
+
+
+def base_noise(noise_type, px, py, fseed, period=(0, 0)):
+    """Single-octave value at one point (float32 inputs)."""
+    return lib().orc_base_noise(noise_type, px, py, fseed, period[0], period[1])
 
 
 def remap(a, vmin=0.0, vmax=1.0):

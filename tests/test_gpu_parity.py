@@ -45,9 +45,11 @@ def test_noise_fbm_options_bitexact(hm, orc):
     assert np.array_equal(g, o)
 
 
-def test_noise_unsupported_type_raises(hm):
+def test_noise_type_outside_the_enum_raises(hm):
     with pytest.raises(hm.HmgError):
-        hm.noise_fbm(hm.NoiseType.SIMPLEX2, (16, 16), (4, 4), 0)
+        hm.noise_fbm(13, (16, 16), (4, 4), 0)
+    with pytest.raises(hm.HmgError):
+        hm.noise(-1, (16, 16), (4, 4), 0)
 
 
 # ---------------------------------------------------------------- thermal
@@ -231,6 +233,25 @@ def test_fused_launch_shapes_do_not_change_results(hm, orc):
         s.close()
     for o in outs[1:]:
         _assert_same(o, outs[0])
+
+
+@pytest.mark.parametrize("storage", [0, 1])
+def test_online_shape_tuning_keeps_results(hm, orc, storage):
+    """A session of 2^20 cells tunes its launch shape on its first
+    iterations (probes across calls); the state equals a fixed-shape run."""
+    z0 = _terrain(orc, 1024, 1024, 12)
+    a = hm.Session(z0, storage=storage, **VP_WORKLOAD)
+    for k in (5, 9, 13, 3):  # the probes straddle calls
+        a.vpipes(k)
+    b = hm.Session(z0, storage=storage, **VP_WORKLOAD)
+    b.set_tuning(64, 1)
+    b.vpipes(30)
+    _assert_same(a.export(), b.export())
+    c = hm.Session(z0, storage=storage, **VP_WORKLOAD)  # shape from the cache
+    c.vpipes(30)
+    _assert_same(c.export(), b.export())
+    for s in (a, b, c):
+        s.close()
 
 
 def test_plain_and_fused_agree_4096(hm, orc):

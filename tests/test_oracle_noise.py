@@ -167,9 +167,13 @@ def test_fbm_period_makes_the_map_tile():
     assert np.array_equal(a[:, : nx // 2], a[:, nx // 2:])
 
 
-def test_unsupported_noise_type():
-    with pytest.raises(ValueError):
-        O.noise_fbm(8, 8, (4.0, 4.0), 0, noise_type=4)
+@pytest.mark.parametrize("noise_type", [0, 5, 8, 11, 12])
+def test_types_without_a_kernel_give_zeros(noise_type):
+    # PARBERRY, SIMPLEX2S, VALUE_DELAUNAY, WORLEY_DOUBLE, WORLEY_VALUE have no
+    # branch in the noise kernels (noise.cl:415-452, 494-576): zeros
+    assert not O.lib().orc_noise_type_supported(noise_type)
+    assert np.all(O.noise_fbm(8, 8, (4.0, 4.0), 3, noise_type=noise_type) == 0)
+    assert np.all(O.noise(8, 8, (4.0, 4.0), 3, noise_type=noise_type) == 0)
 
 
 def test_remap_range_and_flat():
