@@ -82,13 +82,14 @@ def build(verbose=False):
               "-I" + INC, shim, "-o", tmp, "-L" + HERE, "-lhmg",
               "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, SHIM_LIB)
-        drv = os.path.join(ROOT, "tests", "cpp", "shim_driver.cpp")
-        if os.path.exists(drv):
-            out = os.path.join(ROOT, "tests", "cpp", "shim_driver")
-            _run(["g++", "-std=c++17", "-O2", "-I" + INC, drv, "-o", out + ".tmp",
-                  "-L" + HERE, "-lhmap_gpu", "-lhmg",
-                  "-Wl,-rpath,$ORIGIN/../../highmap_amd"])
-            os.replace(out + ".tmp", out)
+        for name in ("shim_driver", "io_driver"):
+            drv = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+            if os.path.exists(drv):
+                out = os.path.join(ROOT, "tests", "cpp", name)
+                _run(["g++", "-std=c++17", "-O2", "-I" + INC, drv, "-o", out + ".tmp",
+                      "-L" + HERE, "-lhmap_gpu", "-lhmg",
+                      "-Wl,-rpath,$ORIGIN/../../highmap_amd"])
+                os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB)
     return LIB

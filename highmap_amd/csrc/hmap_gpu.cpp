@@ -7,6 +7,8 @@
 #include "hmap_gpu.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -30,6 +32,144 @@ float Array::min() const
 float Array::max() const
 {
   return vector.empty() ? 0.f : *std::max_element(vector.begin(), vector.end());
+}
+
+// ------------------------------------------------------------ file formats
+
+void Array::to_file(const std::string &fname) const
+{
+  // Array::to_file (HighMap/src/array/io.cpp:111-121): the floats as stored
+  FILE *f = std::fopen(fname.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot open " + fname + " for writing");
+  const size_t n = std::fwrite(vector.data(), sizeof(float), vector.size(), f);
+  std::fclose(f);
+  if (n != vector.size()) throw std::runtime_error("short write to " + fname);
+}
+
+void Array::from_file(const std::string &fname)
+{
+  // Array::from_file (io.cpp:28-37): size() floats into the current shape
+  FILE *f = std::fopen(fname.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + fname);
+  const size_t n = std::fread(vector.data(), sizeof(float), vector.size(), f);
+  std::fclose(f);
+  if (n != vector.size()) throw std::runtime_error("short read from " + fname);
+}
+
+namespace
+{
+
+// The .npy header dictionary for a float32 (nx, ny) array; padded with
+// spaces and a newline so that the data starts on a 64-byte boundary.
+std::string npy_header(int nx, int ny, bool fortran)
+{
+  std::string d = "{'descr': '<f4', 'fortran_order': ";
+  d += fortran ? "True" : "False";
+  d += ", 'shape': (" + std::to_string(nx) + ", " + std::to_string(ny) + "), }";
+  const size_t pre = 10; // magic (6) + version (2) + header length (2)
+  size_t       total = pre + d.size() + 1;
+  total = (total + 63) / 64 * 64;
+  d.append(total - pre - d.size() - 1, ' ');
+  d += '\n';
+  return d;
+}
+
+// Value text following `key` in the header dictionary.
+std::string npy_field(const std::string &h, const std::string &key)
+{
+  const size_t k = h.find("'" + key + "'");
+  if (k == std::string::npos) throw std::runtime_error(".npy header without " + key);
+  size_t v = h.find(':', k);
+  if (v == std::string::npos) throw std::runtime_error("bad .npy header");
+  v++;
+  while (v < h.size() && h[v] == ' ') v++;
+  size_t e = v;
+  if (h[v] == '(')
+    e = h.find(')', v) + 1;
+  else
+    while (e < h.size() && h[e] != ',' && h[e] != '}') e++;
+  return h.substr(v, e - v);
+}
+
+} // namespace
+
+void Array::to_numpy(const std::string &fname) const
+{
+  // Array::to_numpy (io.cpp:123-131): shape (nx, ny), Fortran order, so the
+  // data is the vector as stored
+  const std::string h = npy_header(shape.x, shape.y, true);
+  FILE             *f = std::fopen(fname.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot open " + fname + " for writing");
+  const unsigned char pre[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+  const uint16_t      hl = (uint16_t)h.size();
+  const unsigned char hlb[2] = {(unsigned char)(hl & 0xff), (unsigned char)(hl >> 8)};
+  bool ok = std::fwrite(pre, 1, 8, f) == 8 && std::fwrite(hlb, 1, 2, f) == 2 &&
+            std::fwrite(h.data(), 1, h.size(), f) == h.size() &&
+            std::fwrite(vector.data(), sizeof(float), vector.size(), f) == vector.size();
+  std::fclose(f);
+  if (!ok) throw std::runtime_error("short write to " + fname);
+}
+
+void Array::from_numpy(const std::string &fname)
+{
+  // Array::from_numpy (io.cpp:39-66): shape (nx, ny) from the file; Fortran
+  // order is the memory order, C order is transposed on the way in
+  FILE *f = std::fopen(fname.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + fname);
+  unsigned char pre[8];
+  std::string   h;
+  bool          ok = std::fread(pre, 1, 8, f) == 8 && std::memcmp(pre + 1, "NUMPY", 5) == 0 &&
+            pre[0] == 0x93;
+  uint32_t hl = 0;
+  if (ok && pre[6] == 1)
+  {
+    unsigned char b[2];
+    ok = std::fread(b, 1, 2, f) == 2;
+    hl = (uint32_t)b[0] | ((uint32_t)b[1] << 8);
+  }
+  else if (ok && (pre[6] == 2 || pre[6] == 3))
+  {
+    unsigned char b[4];
+    ok = std::fread(b, 1, 4, f) == 4;
+    hl = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+  }
+  else
+    ok = false;
+  if (ok)
+  {
+    h.resize(hl);
+    ok = std::fread(&h[0], 1, hl, f) == hl;
+  }
+  if (!ok)
+  {
+    std::fclose(f);
+    throw std::runtime_error(fname + " is not a .npy file");
+  }
+  const std::string descr = npy_field(h, "descr");
+  if (descr != "'<f4'")
+  {
+    std::fclose(f);
+    throw std::runtime_error(fname + ": dtype " + descr + " is not little-endian float32");
+  }
+  const bool        fortran = npy_field(h, "fortran_order") == "True";
+  const std::string sh = npy_field(h, "shape");
+  int               a = 0, b = 0;
+  if (std::sscanf(sh.c_str(), "(%d, %d)", &a, &b) != 2 || a < 0 || b < 0)
+  {
+    std::fclose(f);
+    throw std::runtime_error(fname + ": shape " + sh + " is not 2-D");
+  }
+  std::vector<float> d((size_t)a * (size_t)b);
+  ok = std::fread(d.data(), sizeof(float), d.size(), f) == d.size();
+  std::fclose(f);
+  if (!ok) throw std::runtime_error("short read from " + fname);
+  shape = glm::ivec2(a, b);
+  vector.assign(d.size(), 0.f);
+  if (fortran)
+    vector = std::move(d);
+  else
+    for (int j = 0; j < b; j++)
+      for (int i = 0; i < a; i++) (*this)(i, j) = d[(size_t)i * b + j];
 }
 
 void remap(Array &array, float vmin, float vmax)

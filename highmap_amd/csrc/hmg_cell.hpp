@@ -183,20 +183,42 @@ __device__ __forceinline__ float hash12f(float px, float py, float fseed)
 
 // ----------------------------------------------------------------- thermal
 
-// helper_thermal_exchange (HighMap/src/gpu_opencl/kernels/thermal.cl:5-24).
-__device__ __forceinline__ float th_exchange(float self,
-                                             float other,
-                                             float dist,
-                                             float talus)
+// x / 1.414f, correctly rounded: q0 = x * r with r = RN(1/1.414f), then one
+// fma residual and one fma correction.  Equal to the IEEE quotient for every
+// float x with 2^-100 <= |x| < inf (checked over all 2^32 inputs by
+// tests/cpp/div_const_check.c); other x take the division.
+__device__ __forceinline__ float div_1414(float x)
 {
-  float max_dif = dist * talus;
-  if (self > other)
+  constexpr float B = 1.414f;
+  constexpr float R = 0x1.6a17e6p-1f; // RN(1 / 1.414f)
+  const float     ax = fabsf(x);
+  if (__builtin_expect(ax >= 0x1p-100f && ax < __builtin_inff(), 1))
   {
-    if (self - other > max_dif) return -0.2f * ((self - other) - max_dif) / dist;
-    return 0.f;
+    const float q0 = x * R;
+    return fmaf(fmaf(-q0, B, x), R, q0);
   }
-  if (other - self > max_dif) return 0.2f * ((other - self) - max_dif) / dist;
-  return 0.f;
+  return x / B;
+}
+
+// helper_thermal_exchange (HighMap/src/gpu_opencl/kernels/thermal.cl:5-24),
+// without branches.  With d = self - other (other - self is exactly -d):
+// the reference returns -0.2 (d - c T) / c when d > c T, 0.2 (-d - c T) / c
+// when -d > c T, else 0, i.e. +-(0.2 (|d| - c T)) / c when |d| > c T, with
+// the sign of -d (NaN compares false: 0).  The product and the quotient are
+// rounded as in the reference; negation is exact.  c is 1 (axial, an exact
+// division) or the literal 1.414 (diagonal).
+template <bool DIAG>
+__device__ __forceinline__ float th_exchange(float self, float other, float talus)
+{
+  const float max_dif = (DIAG ? 1.414f : 1.f) * talus;
+  const float d = self - other;
+  const float ad = fabsf(d);
+  const bool  over = ad > max_dif;
+  const float x = 0.2f * (ad - max_dif);
+  // lanes without an exchange divide a harmless 1 (their result is dropped)
+  const float m = DIAG ? div_1414(over ? x : 1.f) : x;
+  const float e = self > other ? -m : m;
+  return over ? e : 0.f;
 }
 
 // Interior thermal update (thermal.cl:41-57): neighbours in the reference's
@@ -206,14 +228,14 @@ __device__ __forceinline__ float th_exchange(float self,
 __device__ __forceinline__ float th_interior(float val, const float n[8], float t)
 {
   float amount = 0.f;
-  amount += th_exchange(val, n[0], 1.f, t);
-  amount += th_exchange(val, n[1], 1.f, t);
-  amount += th_exchange(val, n[2], 1.f, t);
-  amount += th_exchange(val, n[3], 1.f, t);
-  amount += th_exchange(val, n[4], 1.414f, t);
-  amount += th_exchange(val, n[5], 1.414f, t);
-  amount += th_exchange(val, n[6], 1.414f, t);
-  amount += th_exchange(val, n[7], 1.414f, t);
+  amount += th_exchange<false>(val, n[0], t);
+  amount += th_exchange<false>(val, n[1], t);
+  amount += th_exchange<false>(val, n[2], t);
+  amount += th_exchange<false>(val, n[3], t);
+  amount += th_exchange<true>(val, n[4], t);
+  amount += th_exchange<true>(val, n[5], t);
+  amount += th_exchange<true>(val, n[6], t);
+  amount += th_exchange<true>(val, n[7], t);
   return val + amount;
 }
 
@@ -536,12 +558,50 @@ __device__ __forceinline__ Fix128 fix_term(float x)
   return Fix128{lo, (int64_t)hi};
 }
 
+// fix_term for x with bit pattern below 1.0f (finite, sign clear, x < 1):
+// then round-half-even(x * 2^64) < 2^64 and the high word is 0.  Equal to
+// fix_term(x).lo there (same shift, same rounding of the low bits).
+__device__ __forceinline__ uint64_t fix_term_unit(float x)
+{
+  const uint32_t b = __float_as_uint(x);
+  const uint32_t e = b >> 23;
+  const uint32_t m = (b & 0x7fffffu) | 0x800000u;
+  const int      sh = (int)e - 86;
+  if (sh >= 0) return (uint64_t)m << sh; // e in [86, 126]: sh <= 40
+  const uint32_t rsh = (uint32_t)(-sh);
+  if (e == 0 || rsh > 24u) return 0; // zero, subnormal or below 2^-65
+  const uint32_t q = m >> rsh;
+  const uint32_t rem = m & ((1u << rsh) - 1u);
+  const uint32_t half = 1u << (rsh - 1u);
+  return (uint64_t)(q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u));
+}
+
+// a += t for a term below 2^64: one add with carry into the high word.
+__device__ __forceinline__ void fix_add_u64(Fix128 &a, uint64_t t)
+{
+  unsigned long long carry = 0;
+  a.lo = __builtin_addcll(a.lo, t, 0ull, &carry);
+  a.hi = (int64_t)((uint64_t)a.hi + carry);
+}
+
 __device__ __forceinline__ void fix_add(Fix128 &a, const Fix128 &b)
 {
   uint64_t lo = a.lo + b.lo;
   uint64_t c = lo < a.lo ? 1u : 0u;
   a.lo = lo;
   a.hi = (int64_t)((uint64_t)a.hi + (uint64_t)b.hi + c);
+}
+
+// acc += fix_term(x): the short form when every active lane of the wave holds
+// a finite x in [0, 1) (water depths in height units), the general one
+// otherwise.  The vote is uniform over the wave, so the branch does not
+// diverge.
+__device__ __forceinline__ void fix_accumulate(Fix128 &a, float x)
+{
+  if (__all(__float_as_uint(x) < 0x3f800000u))
+    fix_add_u64(a, fix_term_unit(x));
+  else
+    fix_add(a, fix_term(x));
 }
 
 __host__ __device__ __forceinline__ double fix_to_double(uint64_t lo, int64_t hi)

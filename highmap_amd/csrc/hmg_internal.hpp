@@ -121,6 +121,10 @@ VpConst make_vp_const(const hmg_vpipes_params &p, int nx, int ny);
 int validate_vp_params(const hmg_vpipes_params &p);
 
 // Kernel-side entry points implemented in the .hip sources.
+// Runs `iterations` Jacobi thermal steps on z, ping-ponging with zscratch.
+// With result == nullptr the result ends in z (copied back after an odd
+// number of launches); otherwise *result receives whichever of the two
+// planes holds it and nothing is copied.
 int launch_thermal(hmg_ctx     *ctx,
                    float       *z,
                    float       *zscratch,
@@ -130,7 +134,8 @@ int launch_thermal(hmg_ctx     *ctx,
                    float        talus_scalar,
                    int          iterations,
                    const float *bedrock,
-                   hipStream_t  st);
+                   hipStream_t  st,
+                   float      **result = nullptr);
 int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
 
 // Per-block min and max of a[0..n) into mm[2 + 2b], mm[3 + 2b] for the nblk

@@ -145,13 +145,16 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
       int k = j->desc.iterations - j->done;
       if (k > 8) k = 8;
       hmg_session *s = j->s;
-      int          r2 = launch_thermal(ctx, s->z[s->cur].as<float>(), s->z[s->cur ^ 1].as<float>(),
-                                       s->nx, s->ny, nullptr, j->desc.talus, k, nullptr, ctx->stream);
+      float       *zc = s->z[s->cur].as<float>(), *res = zc;
+      int          r2 = launch_thermal(ctx, zc, s->z[s->cur ^ 1].as<float>(), s->nx, s->ny,
+                                       nullptr, j->desc.talus, k, nullptr, ctx->stream, &res);
       if (r2 != HMG_OK)
       {
         j->state = r2;
         continue;
       }
+      // an odd number of launches leaves the result in the other plane
+      if (res != zc) std::swap(s->z[s->cur].ptr, s->z[s->cur ^ 1].ptr);
       j->done += k;
       work += (double)j->desc.nx * j->desc.ny * k;
       if (j->done == j->desc.iterations)

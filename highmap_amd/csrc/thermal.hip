@@ -292,8 +292,8 @@ int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st)
 // launch.  Exposed for the tuner.
 static int g_thermal_block_steps = 8;
 
-// Runs `iterations` Jacobi steps.  z holds the input and receives the
-// result; zscratch is a second plane of the same size.
+// Runs `iterations` Jacobi steps.  z holds the input; zscratch is a second
+// plane of the same size (see the declaration for where the result ends).
 int launch_thermal(hmg_ctx     *ctx,
                    float       *z,
                    float       *zscratch,
@@ -303,7 +303,8 @@ int launch_thermal(hmg_ctx     *ctx,
                    float        talus_scalar,
                    int          iterations,
                    const float *bedrock,
-                   hipStream_t  st)
+                   hipStream_t  st,
+                   float      **result)
 {
   (void)ctx;
   float *a = z, *b = zscratch;
@@ -334,7 +335,9 @@ int launch_thermal(hmg_ctx     *ctx,
     a = b;
     b = t;
   }
-  if (a != z)
+  if (result)
+    *result = a;
+  else if (a != z)
     HMG_HIP(hipMemcpyAsync(z, a, (size_t)nx * ny * sizeof(float), hipMemcpyDeviceToDevice, st));
   return HMG_OK;
 }

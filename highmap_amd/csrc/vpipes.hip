@@ -542,7 +542,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
         o.t = S::st(c_0[2]);
         o.b = S::st(c_0[3]);
         at_off(Fo, c) = o;
-        if (k.maintain) fix_add(acc, fix_term(d2 * k.ce));
+        if (k.maintain) fix_accumulate(acc, d2 * k.ce);
       }
     }
   }
@@ -1211,9 +1211,13 @@ int hmg_session_thermal(hmg_session *s, float talus, int iterations)
   HMG_TRY(ctx_enter(s->ctx));
   float *zc = s->z[s->cur].as<float>();
   float *zt = s->z[s->cur ^ 1].as<float>();
+  float *res = zc;
   HMG_TRY(launch_thermal(s->ctx, zc, zt, s->nx, s->ny, nullptr, talus, iterations, nullptr,
-                         s->ctx->stream));
-  return launch_extrapolate_borders(zc, s->nx, s->ny, s->ctx->stream);
+                         s->ctx->stream, &res));
+  // the other z plane is free between iterations: after an odd number of
+  // launches the result is there, and the two planes trade places
+  if (res != zc) std::swap(s->z[s->cur].ptr, s->z[s->cur ^ 1].ptr);
+  return launch_extrapolate_borders(s->z[s->cur].as<float>(), s->nx, s->ny, s->ctx->stream);
 }
 
 int hmg_session_export(hmg_session *s, float *z_host, float *d_host, float *s_host,

@@ -21,6 +21,7 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #if __has_include(<glm/glm.hpp>)
@@ -87,6 +88,18 @@ public:
   float        sum() const;
   float        min() const;
   float        max() const;
+
+  // Interchange formats (HighMap/src/array/io.cpp:28-66,111-131).
+  // Raw float32 in vector order (j*nx + i); from_file reads size() floats
+  // into the current shape.
+  void to_file(const std::string &fname) const;
+  void from_file(const std::string &fname);
+  // .npy: written as float32 of shape (nx, ny) in Fortran order (so element
+  // (i, j) is at j*nx + i, as in memory); read back in either order, the
+  // shape taken from the file.  Throws std::runtime_error on I/O errors or
+  // a dtype other than little-endian float32.
+  void to_numpy(const std::string &fname) const;
+  void from_numpy(const std::string &fname);
 };
 
 // hmap::remap (HighMap/src/range/range.cpp:342-361).
