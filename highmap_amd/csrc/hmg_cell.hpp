@@ -423,6 +423,9 @@ __device__ __forceinline__ void mirror_axis(float p,
 // The transport sample split in two: `issue` computes the weights and loads
 // the four texels, `finish` forms the weighted sum, so that other work can
 // sit between the loads and their use.  Same operations as vp_transport.
+// With `direct` set, all four texels are read at (i, j) instead (the caller
+// then uses s00 alone): selects, not a branch, so that the same loads are in
+// flight on both paths.
 struct TexelFetch
 {
   float w00, w10, w01, w11;
@@ -435,7 +438,8 @@ __device__ __forceinline__ TexelFetch vp_transport_issue(const typename S::T *sp
                                                          int                  j,
                                                          float                u,
                                                          float                v,
-                                                         const VpConst       &k)
+                                                         const VpConst       &k,
+                                                         bool                 direct = false)
 {
   float      px = (float)i - k.dt * u;
   float      py = (float)j - k.dt * v;
@@ -444,6 +448,10 @@ __device__ __forceinline__ TexelFetch vp_transport_issue(const typename S::T *sp
   TexelFetch t;
   mirror_axis(px, k.nx, k.inv_nx, k.pow2x, i0, i1, a);
   mirror_axis(py, k.ny, k.inv_ny, k.pow2y, j0, j1, b);
+  i0 = direct ? i : i0;
+  i1 = direct ? i : i1;
+  j0 = direct ? j : j0;
+  j1 = direct ? j : j1;
   t.w00 = (1.f - a) * (1.f - b);
   t.w10 = a * (1.f - b);
   t.w01 = (1.f - a) * b;

@@ -415,20 +415,13 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   };
   // B(r): transport of row r from A(r-1), A(r), A(r+1), in two halves
   auto transport_issue = [&](const RowA &am, const RowA &a, const RowA &ap, int r) {
-    const int  gj = clampi(r, ny);
-    TexelFetch t;
-    if (!y.has_prev)
-    {
-      t.w00 = 1.f;
-      t.w10 = t.w01 = t.w11 = 0.f;
-      t.s00 = S::ld(at_off(Sp, (uint32_t)(gj * nx + gi)));
-      t.s10 = t.s01 = t.s11 = 0.f;
-      return t;
-    }
-    float frl = from_left(a.fr), flr = from_right(a.fl);
-    float u, v;
+    const int gj = clampi(r, ny);
+    float     frl = from_left(a.fr), flr = from_right(a.fl);
+    float     u, v;
     vp_velocity(a.dr, frl, am.ft, flr, ap.fb, a.fl, a.fr, a.ft, a.fb, u, v, k);
-    return vp_transport_issue<S>(Sp, gi, gj, u, v, k);
+    // first iteration of a session: the stored sediment at (gi, gj), read
+    // without a branch so that the loads in flight do not depend on it
+    return vp_transport_issue<S>(Sp, gi, gj, u, v, k, !y.has_prev);
   };
   auto transport_finish = [&](const TexelFetch &t) {
     // first iteration of a session: the stored sediment, untransported
@@ -483,17 +476,16 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
       if (r >= j1) break;
       RowA &am1 = R[q % U], &a0 = R[(q + 1) % U], &ap1 = R[(q + 2) % U];
       RowA &ap2 = R[(q + 3) % U];
-      // rows are loaded one row before they are used with the load-ahead
-      if (PF)
-        R[(q + 4) % U] = load(r + 3);
-      else
-        ap2 = load(r + 2);
+      // without the load-ahead the row is loaded here; with it, after this
+      // row's gathers (below)
+      if (!PF) ap2 = load(r + 2);
       float &s_m1 = Sr[q % U], &s_0 = Sr[(q + 1) % U], &s_p1 = Sr[(q + 2) % U];
       float *c_m1 = Cr[q % U], *c_0 = Cr[(q + 1) % U], *c_p1 = Cr[(q + 2) % U];
 
       if (FLOW)
       {
         // flow_simulation: the water pass of row r, then D and F
+        if (PF) R[(q + 4) % U] = load(r + 3);
         flow(a0, ap1, ap2, c_p1);
         const bool top = r == 0, bot = r == ny - 1;
         float      ftb = top ? c_0[2] : c_m1[2], fbt = bot ? c_0[3] : c_p1[3];
@@ -513,8 +505,23 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
         }
         continue;
       }
+      // Memory order within the row: the sediment gathers of row r+1 first,
+      // then the load-ahead row r+3; the flow of row r+1 runs while both are
+      // in flight, then the gathers are consumed.  Loads complete in issue
+      // order for the wait counter, so with the row loads issued after the
+      // gathers, waiting for the gathers leaves the row loads in flight (to
+      // be consumed one row later).  The empty asm statements fix this order
+      // against the scheduler: loads cannot cross a memory clobber, the flow
+      // starts from values redefined after the loads, and the gathered
+      // samples are first used after the flow results.
       TexelFetch tf = transport_issue(a0, ap1, ap2, r + 1);
+      asm volatile("" ::: "memory");
+      if (PF) R[(q + 4) % U] = load(r + 3);
+      asm volatile("" : "+v"(ap1.z), "+v"(ap1.d1)::"memory");
       flow(a0, ap1, ap2, c_p1);
+      asm volatile(""
+                   : "+v"(c_p1[0]), "+v"(c_p1[1]), "+v"(c_p1[2]), "+v"(c_p1[3]), "+v"(tf.s00),
+                     "+v"(tf.s10), "+v"(tf.s01), "+v"(tf.s11));
       s_p1 = transport_finish(tf);
 
       // row r: water pass and erosion
