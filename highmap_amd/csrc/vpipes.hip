@@ -40,6 +40,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "hmg_internal.hpp"
 
@@ -276,6 +277,7 @@ __global__ void __launch_bounds__(256) k_vp_sum(const typename S::T *__restrict_
 // and B/C(ny) are replaced by those of row 0 and ny-1.
 
 constexpr int STRIP_OUT = 60; // columns written per wavefront
+constexpr int WPB = 4;        // wavefronts (adjacent strips) per workgroup
 
 // lane i receives lane i-1's value (wave_shr:1)
 __device__ __forceinline__ float from_left(float v)
@@ -360,7 +362,7 @@ __device__ __forceinline__ int xcd_wave_item()
 {
   const int G = gridDim.x;
   const int b = (G & 7) ? blockIdx.x : (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
-  return __builtin_amdgcn_readfirstlane(b * 4 + (threadIdx.x >> 6));
+  return __builtin_amdgcn_readfirstlane(b * WPB + (threadIdx.x >> 6));
 }
 
 // One wavefront's work item of one iteration of one map: a strip of 60
@@ -449,7 +451,8 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   // an unrolled pass finds A(r+q-1) in slot q, A(r+q) in q+1, ..., and
   // B/C(r+q-1), B/C(r+q), B/C(r+q+1) in slots q, q+1, q+2 (mod U).  With
   // the load-ahead row, five A rows are live at once.
-  constexpr int U = PF ? 5 : 4;
+  constexpr int  U = PF ? 5 : 4;
+  constexpr bool ROW_SYNC = !std::is_same<S, StBF16>::value;
   RowA  R[U];
   float Sr[U], Cr[U][4];
 
@@ -474,6 +477,13 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
     {
       const int r = r0 + q;
       if (r >= j1) break;
+      // The wavefronts of a workgroup (adjacent strips) step through the
+      // rows together, so the cache lines two strips share at their edge
+      // are fetched from memory once rather than once per strip.  A
+      // wavefront that has left the loop or the kernel no longer counts at
+      // the barrier.  Measured at 4096^2: FP32 and FP16 state 2-7% faster
+      // with it, BF16 about 3% slower.
+      if (ROW_SYNC) __builtin_amdgcn_s_barrier();
       RowA &am1 = R[q % U], &a0 = R[(q + 1) % U], &ap1 = R[(q + 2) % U];
       RowA &ap2 = R[(q + 3) % U];
       // without the load-ahead the row is loaded here; with it, after this
@@ -575,7 +585,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
 
 // One iteration of one session.
 template <class S, bool URAIN, bool PF, bool KEXP1>
-__global__ void __launch_bounds__(256) k_vp_stream(const float *__restrict__ z,
+__global__ void __launch_bounds__(64 * WPB) k_vp_stream(const float *__restrict__ z,
                                                    const typename S::T *__restrict__ D,
                                                    const Flux4<S> *__restrict__ F,
                                                    const typename S::T *__restrict__ Sp,
@@ -622,7 +632,7 @@ struct VpMapDesc
 };
 
 template <class S, bool URAIN, bool PF, bool KEXP1>
-__global__ void __launch_bounds__(256) k_vp_batch(const VpMapDesc *__restrict__ maps,
+__global__ void __launch_bounds__(64 * WPB) k_vp_batch(const VpMapDesc *__restrict__ maps,
                                                   int n_maps,
                                                   int n_items,
                                                   int t)
@@ -860,7 +870,7 @@ static int session_step_fused(hmg_session *s)
   const int   strips = (nx + STRIP_OUT - 1) / STRIP_OUT;
   const int   ch = s->chunk_rows;
   const int   n_items = strips * ((ny + ch - 1) / ch);
-  const int   G = (n_items + 3) / 4;
+  const int   G = (n_items + WPB - 1) / WPB;
   VpDyn       y = make_dyn(s);
   const T    *D = (const T *)s->D[a].ptr;
   const T    *Sp = (const T *)s->Sp[a].ptr;
@@ -873,7 +883,7 @@ static int session_step_fused(hmg_session *s)
   Fix128      *cset = slot_set(s, s->iter + 1);
   const float *z = s->z[a].as<float>();
 #define HMG_STREAM(UR, PFV, KE)                                                \
-  k_vp_stream<S, UR, PFV, KE><<<G, 256, 0, st>>>(z, D, F, Sp, rain, zo, Do, Fo, So, pset, \
+  k_vp_stream<S, UR, PFV, KE><<<G, 64 * WPB, 0, st>>>(z, D, F, Sp, rain, zo, Do, Fo, So, pset, \
                                                  aset, cset, strips, ch, n_items, s->k, y)
   const bool ke1 = s->k.kexp == 1.f;
   if (rain)
@@ -1461,13 +1471,13 @@ static int batch_advance_t(hmg_ctx           *ctx,
   {
     while (active > 0 && d[active - 1].iters <= t) active--;
     const int items = prefix[active];
-    const int G = (items + 3) / 4;
+    const int G = (items + WPB - 1) / WPB;
     if (G == 0) continue;
     const VpMapDesc *tab = table.as<VpMapDesc>();
     if (any_rain)
-      k_vp_batch<S, false, PF, KEXP1><<<G, 256, 0, st>>>(tab, active, items, t);
+      k_vp_batch<S, false, PF, KEXP1><<<G, 64 * WPB, 0, st>>>(tab, active, items, t);
     else
-      k_vp_batch<S, true, PF, KEXP1><<<G, 256, 0, st>>>(tab, active, items, t);
+      k_vp_batch<S, true, PF, KEXP1><<<G, 64 * WPB, 0, st>>>(tab, active, items, t);
   }
   int rc = check_launch("vp_batch");
   if (use_graph)
@@ -1731,7 +1741,7 @@ namespace hmg
 {
 
 template <bool PF>
-__global__ void __launch_bounds__(256) k_flow_stream(const float *__restrict__ z,
+__global__ void __launch_bounds__(64 * WPB) k_flow_stream(const float *__restrict__ z,
                                                      const float *__restrict__ D,
                                                      const Flux4<StF32> *__restrict__ F,
                                                      float *__restrict__ Do,
@@ -1843,11 +1853,11 @@ int hmg_flow_simulation_dev(hmg_ctx     *ctx,
   long       ch = ((long)ny * strips + target - 1) / target;
   ch = std::max(8L, std::min(64L, ch));
   const int n_items = strips * (int)((ny + ch - 1) / ch);
-  const int G = (n_items + 3) / 4;
+  const int G = (n_items + WPB - 1) / WPB;
   int       a = 0;
   for (int it = 0; it < iterations; it++, a ^= 1)
   {
-    k_flow_stream<true><<<G, 256, 0, st>>>(z_dev, D[a].as<float>(),
+    k_flow_stream<true><<<G, 64 * WPB, 0, st>>>(z_dev, D[a].as<float>(),
                                            (const Flux4<StF32> *)F[a].ptr, D[a ^ 1].as<float>(),
                                            (Flux4<StF32> *)F[a ^ 1].ptr, strips, (int)ch,
                                            n_items, k);
