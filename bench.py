@@ -9,7 +9,7 @@ k_capacity 0.1, k_erode 0.05, k_depose 0.05, evaporation 0.01).  The default
 50 timed steps are one full 500-iteration job.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--storage f32|bf16|f16]
-    python bench.py --workload batch|serve|precision|thermal256|vpipes1024|copy
+    python bench.py --workload batch|serve|precision|thermal256|vpipes1024|flow|snow|copy
 
 Each run prints one JSON line.  For the headline, `value` is hydraulic
 cell-iterations per second (nx*ny*iterations / wall time of the timed steps,
@@ -457,6 +457,75 @@ def run_flow(args):
                              "sample": "4096^2, 10 iterations, median of 3"}}
 
 
+def run_snow(args):
+    """hmap::gpu::snow_simulation.  (i) The example configuration
+    (examples/ex_snow_simulation: 256^2 fBm seed 2, snow 0.1, uniform fall,
+    no melting, talus 2.5/256, 500 iterations, reference defaults with the
+    thermal post-filter) as drop-in calls on host arrays, checked bit for
+    bit against the FP32 oracle.  (ii) 4096^2, 100 iterations, device
+    resident without the post-filter, for the step kernel's roofline at
+    24 B per cell-iteration (z, s, talus, melting map and fall map read, s
+    written)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from oracle import pyoracle as O
+
+    torch, hm, ctx = _gpu()
+    n0, it0 = 256, 500
+    z0 = O.fbm_terrain(n0, n0, 2)
+    talus0 = np.float32(2.5) / np.float32(n0)
+    hm.snow_simulation(z0, 0.1, 1.0, 0.0, talus0, 5)  # warm-up
+    best_dropin = 1e9
+    for _ in range(3):
+        t0 = time.perf_counter()
+        got = hm.snow_simulation(z0, 0.1, 1.0, 0.0, talus0, it0)
+        best_dropin = min(best_dropin, time.perf_counter() - t0)
+    print("snow: GPU example done, running the oracle", file=sys.stderr, flush=True)
+    want = O.snow_simulation(z0, 0.1, 1.0, 0.0, talus0, it0)
+
+    n, iters = args.n, 100
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    z = device_terrain(torch, hm, ctx, n, n, 2)
+    fall = torch.ones_like(z)
+    melt = torch.zeros_like(z)
+    tal = torch.full_like(z, 2.5 / n)
+    s = torch.empty_like(z)
+
+    def resident(it):
+        hm._check(hm.lib().hmg_snow_simulation_dev(
+            ctx.handle, z.data_ptr(), n, n, 0.1, fall.data_ptr(), melt.data_ptr(),
+            tal.data_ptr(), it, 0.5, 1.0, 0.5, 0.1, 0.8, 1.0, 1.0, 0, 0.2, s.data_ptr()))
+
+    resident(3)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    best = 1e9
+    for _ in range(3):
+        e0.record(stream)
+        resident(iters)
+        e1.record(stream)
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) / 1e3)
+    c = cpu_bench("snow", n0, n0, it0, 3)
+    alg = 24 * n * n * iters
+    return {"metric": "Gcell-iter/s snow_simulation 256^2 x 500 (example)",
+            "unit": "Gcell-iter/s", "n_gpus": 1,
+            "value": round(n0 * n0 * it0 / best_dropin / 1e9, 3),
+            "bitexact_vs_fp32_oracle": bool(np.array_equal(got, want)),
+            "resident_4096_gcell_iter_s": round(n * n * iters / best / 1e9, 3),
+            "roofline": {"kernel": "k_snow_step", "bound": "hbm",
+                         "achieved": round(alg / best / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / best / 1e9 / HBM_PEAK_GBS, 4),
+                         "avg_launch_us": round(best / iters * 1e6, 2),
+                         "note": "4096^2 resident run, post-filter off; time per iteration "
+                                 "includes the call's fixed work spread over 100 iterations"},
+            "cpu_baseline": {"value": round(c["gcell_iter_s"], 5), "unit": "Gcell-iter/s",
+                             "cores": c["threads"], "kind": "port",
+                             "sample": "the example configuration, 256^2 x 500, median of 3"}}
+
+
 def run_vpipes1024(args):
     """hydraulic_vpipes at 1024^2 (seed 2), 200 iterations: (i) reference
     semantics (gravity 1, volume top-up) with plain and fused launches,
@@ -540,7 +609,7 @@ def main():
                     help="serve: cells of the jobs that run at once (0: no cap)")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batch", "serve", "precision", "thermal256",
-                             "vpipes1024", "flow", "copy"])
+                             "vpipes1024", "flow", "snow", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -552,7 +621,7 @@ def main():
         sys.exit(2)
     fn = {"headline": run_headline, "batch": run_batch, "precision": run_precision,
           "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "flow": run_flow,
-          "copy": run_copy}
+          "snow": run_snow, "copy": run_copy}
     if args.workload == "serve":
         from highmap_amd import serve
 

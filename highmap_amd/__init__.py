@@ -160,6 +160,10 @@ def lib():
                                     i32),
             "hmg_flow_simulation_dev": ([vp, vp, i32, i32, f32, vp, i32, f32, i32, f32, f32,
                                          vp], i32),
+            "hmg_snow_simulation": ([vp, vp, i32, i32, f32, vp, vp, vp, i32] + [f32] * 7
+                                    + [i32, f32, vp], i32),
+            "hmg_snow_simulation_dev": ([vp, vp, i32, i32, f32, vp, vp, vp, i32] + [f32] * 7
+                                        + [i32, f32, vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name, None)
@@ -395,6 +399,32 @@ def flow_simulation(z, water_height, depth_map, iterations, dt=0.5, flux_diffusi
                                      int(bool(flux_diffusion)), float(flux_diffusion_strength),
                                      float(dry_out_ratio), _ptr(d)))
     return d
+
+
+def snow_simulation(z, snow_depth, fall_map, melting_map, talus, iterations, dt=0.5,
+                    fall_iterations_ratio=1.0, k_snow=0.5, k_visc=0.1, k_melt_factor=0.8,
+                    k_depth_ratio=1.0, k_depth_slope_ratio=1.0, post_filter=True,
+                    thermal_talus_ratio=0.2, ctx=None):
+    """hmap::gpu::snow_simulation (hydrology/snow_simulation.cpp:81-154):
+    returns the snow depth after `iterations` steps of fall, avalanching,
+    viscosity and melting, then the optional thermal post-filter.
+    fall_map, melting_map and talus are planes of z's shape (scalars are
+    broadcast); the inputs are not changed."""
+    zz = _f32(z, None, "z")
+    ny, nx = zz.shape
+
+    def plane(a, name):
+        return _f32(np.broadcast_to(np.asarray(a, np.float32), (ny, nx)), (ny, nx), name)
+
+    f, m, t = plane(fall_map, "fall_map"), plane(melting_map, "melting_map"), plane(talus, "talus")
+    out = np.empty((ny, nx), np.float32)
+    _check(lib().hmg_snow_simulation(_ctx(ctx).handle, _ptr(zz), nx, ny, float(snow_depth),
+                                     _ptr(f), _ptr(m), _ptr(t), int(iterations), float(dt),
+                                     float(fall_iterations_ratio), float(k_snow), float(k_visc),
+                                     float(k_melt_factor), float(k_depth_ratio),
+                                     float(k_depth_slope_ratio), int(bool(post_filter)),
+                                     float(thermal_talus_ratio), _ptr(out)))
+    return out
 
 
 class Session:

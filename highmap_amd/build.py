@@ -25,7 +25,7 @@ LIB = os.path.join(HERE, "libhmg.so")
 SHIM_LIB = os.path.join(HERE, "libhmap_gpu.so")
 ARCH = os.environ.get("HMG_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["runtime.hip", "noise.hip", "thermal.hip", "vpipes.hip", "server.hip"]
+HIP_SOURCES = ["runtime.hip", "noise.hip", "thermal.hip", "vpipes.hip", "server.hip", "snow.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=" + ARCH,

@@ -524,5 +524,49 @@ Array flow_simulation(const Array &z,
   return d;
 }
 
+// HighMap/src/hydrology/snow_simulation.cpp:81-154
+Array snow_simulation(const Array &z,
+                      float        snow_depth,
+                      const Array &fall_map,
+                      const Array &melting_map,
+                      const Array &talus,
+                      int          iterations,
+                      float        dt,
+                      float        fall_iterations_ratio,
+                      float        k_snow,
+                      float        k_visc,
+                      float        k_melt_factor,
+                      float        k_depth_ratio,
+                      float        k_depth_slope_ratio,
+                      bool         post_filter,
+                      float        thermal_talus_ratio)
+{
+  check_shape(fall_map, z.shape, "fall_map");
+  check_shape(melting_map, z.shape, "melting_map");
+  check_shape(talus, z.shape, "talus");
+  Array s(z.shape);
+  check(hmg_snow_simulation(ctx(),
+                            z.vector.data(),
+                            z.shape.x,
+                            z.shape.y,
+                            snow_depth,
+                            fall_map.vector.data(),
+                            melting_map.vector.data(),
+                            talus.vector.data(),
+                            iterations,
+                            dt,
+                            fall_iterations_ratio,
+                            k_snow,
+                            k_visc,
+                            k_melt_factor,
+                            k_depth_ratio,
+                            k_depth_slope_ratio,
+                            post_filter ? 1 : 0,
+                            thermal_talus_ratio,
+                            s.vector.data()),
+        "snow_simulation");
+  return s;
+}
+
 } // namespace gpu
 } // namespace hmap

@@ -148,6 +148,31 @@ __device__ __forceinline__ void hm_sincos(float x, float &sn, float &cs)
   cs = ((q + 1) & 2) ? -cs : cs;
 }
 
+// Project-written exp (OpenCL's exp, used by snow_simulation.cl, is not
+// correctly rounded): x = k ln2 + r with k = rint(x * RN(1/ln2)), r by two
+// fmaf steps (ln2 as its nearest float plus the remainder), e^r by the
+// degree-7 Taylor polynomial in Horner form with fmaf, then times 2^k as
+// two exact powers of two so that only the last product rounds (into the
+// subnormal range for results that small).  About 1 ulp.
+__device__ __forceinline__ float hm_exp(float x)
+{
+  if (x != x) return x;
+  if (x > 89.f) return __builtin_inff();
+  if (x < -104.f) return 0.f;
+  const float k = __builtin_rintf(x * 0x1.715476p+0f);
+  float       r = fmaf(-k, 0x1.62e430p-1f, x);
+  r = fmaf(-k, -0x1.05c610p-29f, r);
+  float p = fmaf(r, 1.f / 5040.f, 1.f / 720.f);
+  p = fmaf(r, p, 1.f / 120.f);
+  p = fmaf(r, p, 1.f / 24.f);
+  p = fmaf(r, p, 1.f / 6.f);
+  p = fmaf(r, p, 0.5f);
+  p = fmaf(r, p, 1.f);
+  p = fmaf(r, p, 1.f);
+  const int ki = (int)k, k1 = ki / 2, k2 = ki - k1;
+  return (p * __int_as_float((k1 + 127) << 23)) * __int_as_float((k2 + 127) << 23);
+}
+
 // OpenCL fract(): min(x - floor(x), 0x1.fffffep-1f).
 __device__ __forceinline__ float fract_cl(float x)
 {

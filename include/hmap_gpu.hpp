@@ -9,6 +9,7 @@
 //   hmap::gpu::sediment_deposition (2)  HighMap/include/highmap/erosion.hpp:1366-1379
 //   hmap::gpu::hydraulic_vpipes   HighMap/include/highmap/erosion.hpp:1206-1223
 //   hmap::gpu::flow_simulation    HighMap/include/highmap/hydrology/hydrology.hpp:875-882
+//   hmap::gpu::snow_simulation    HighMap/include/highmap/hydrology/hydrology.hpp:942-956
 // plus the minimal hmap::Array (HighMap/include/highmap/array.hpp:31-44,
 // 236-239: float storage, shape {nx, ny}, index j*nx + i) and hmap::NoiseType
 // (HighMap/include/highmap/functions.hpp:66-81) those signatures use.
@@ -214,6 +215,22 @@ Array flow_simulation(const Array &z,
                       bool         flux_diffusion = true,
                       float        flux_diffusion_strength = 0.01f,
                       float        dry_out_ratio = 0.f);
+
+Array snow_simulation(const Array &z,
+                      float        snow_depth,
+                      const Array &fall_map,
+                      const Array &melting_map,
+                      const Array &talus,
+                      int          iterations,
+                      float        dt = 0.5f,
+                      float        fall_iterations_ratio = 1.f,
+                      float        k_snow = 0.5f,
+                      float        k_visc = 0.1f,
+                      float        k_melt_factor = 0.8f,
+                      float        k_depth_ratio = 1.f,
+                      float        k_depth_slope_ratio = 1.f,
+                      bool         post_filter = true,
+                      float        thermal_talus_ratio = 0.2f);
 
 } // namespace gpu
 } // namespace hmap

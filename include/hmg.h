@@ -331,6 +331,56 @@ int hmg_flow_simulation_dev(hmg_ctx     *ctx,
                             float        dry_out_ratio,
                             float       *d_out_dev);
 
+/* ---------------------------------------------------- snow simulation -- */
+
+/* hmap::gpu::snow_simulation (HighMap/src/hydrology/snow_simulation.cpp:
+ * 81-154, kernel HighMap/src/gpu_opencl/kernels/snow_simulation.cl; defaults
+ * in include/highmap/hydrology/hydrology.hpp:942-956: dt 0.5,
+ * fall_iterations_ratio 1, k_snow 0.5, k_visc 0.1, k_melt_factor 0.8,
+ * k_depth_ratio 1, k_depth_slope_ratio 1, post_filter 1,
+ * thermal_talus_ratio 0.2).  fall_map, melting_map and talus are nx*ny
+ * planes.  Writes the snow depth (nx*ny floats) to s_out_host; the inputs
+ * are not changed.  nx, ny >= 3, iterations >= 0. */
+int hmg_snow_simulation(hmg_ctx     *ctx,
+                        const float *z_host,
+                        int          nx,
+                        int          ny,
+                        float        snow_depth,
+                        const float *fall_map_host,
+                        const float *melting_map_host,
+                        const float *talus_host,
+                        int          iterations,
+                        float        dt,
+                        float        fall_iterations_ratio,
+                        float        k_snow,
+                        float        k_visc,
+                        float        k_melt_factor,
+                        float        k_depth_ratio,
+                        float        k_depth_slope_ratio,
+                        int          post_filter,
+                        float        thermal_talus_ratio,
+                        float       *s_out_host);
+/* The same on device pointers (s_out_dev may not alias the inputs). */
+int hmg_snow_simulation_dev(hmg_ctx     *ctx,
+                            const float *z_dev,
+                            int          nx,
+                            int          ny,
+                            float        snow_depth,
+                            const float *fall_map_dev,
+                            const float *melting_map_dev,
+                            const float *talus_dev,
+                            int          iterations,
+                            float        dt,
+                            float        fall_iterations_ratio,
+                            float        k_snow,
+                            float        k_visc,
+                            float        k_melt_factor,
+                            float        k_depth_ratio,
+                            float        k_depth_slope_ratio,
+                            int          post_filter,
+                            float        thermal_talus_ratio,
+                            float       *s_out_dev);
+
 /* ------------------------------------------------------------ session -- */
 
 /* A session keeps one erosion state (z, water, sediment, fluxes) resident

@@ -11,6 +11,9 @@
  *             thermal         thermal only (talus 2/nx)
  *             flow            flow_simulation (water_height 1e-2, uniform
  *                             depth map, dt 0.5, flux diffusion 0.01)
+ *             snow            snow_simulation as ex_snow_simulation (snow
+ *                             0.1, uniform fall, no melting, talus 2.5/nx,
+ *                             defaults, post-filter on)
  * Input: remap(noise_fbm(PERLIN, kw 4, seed 2)) to [0,1] (not timed).
  * Output: one JSON line with the median wall time of `reps` timed runs after
  * one warm-up run.
@@ -87,6 +90,18 @@ int main(int argc, char **argv)
     {
       for (size_t k = 0; k < n; k++) u[k] = 1.f; /* depth map */
       orc_flow_simulation_f32(z, nx, ny, 1e-2f, u, iters, 0.5f, 1, 0.01f, 0.f, d);
+      z[n / 2] = d[n / 2]; /* checksum below */
+    }
+    else if (!strcmp(wl, "snow"))
+    {
+      for (size_t k = 0; k < n; k++)
+      {
+        u[k] = 1.f;              /* fall map */
+        v[k] = 0.f;              /* melting map */
+        fl[k] = 2.5f / (float)nx; /* talus */
+      }
+      orc_snow_simulation_f32(z, nx, ny, 0.1f, u, v, fl, iters, 0.5f, 1.f, 0.5f, 0.1f, 0.8f,
+                              1.f, 1.f, 1, 0.2f, d);
       z[n / 2] = d[n / 2]; /* checksum below */
     }
     else if (!strcmp(wl, "thermal"))

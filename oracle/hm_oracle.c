@@ -127,6 +127,43 @@ float orc_cosf(float x)
   }
 }
 
+/* 2^k as a float for k in [-126, 127] (exact). */
+static float orc_pow2i(int k)
+{
+  uint32_t b = (uint32_t)(k + 127) << 23;
+  float    f;
+  memcpy(&f, &b, sizeof f);
+  return f;
+}
+
+/* exp(x) of the FP32 contract.  OpenCL's exp (used by snow_simulation.cl)
+ * is not correctly rounded, so the kernels and this restatement share one
+ * function: x = k ln2 + r with k = rint(x / ln2) (a product by the float
+ * nearest 1/ln2) and r from two fma steps (ln2 split into its nearest float
+ * and the remainder), e^r from the degree-7 Taylor polynomial in Horner
+ * form with fma, then times 2^k as two exact powers of two, so that only
+ * the last product rounds (into the subnormal range when the result is
+ * that small).  Within about 1 ulp of exp; pinned against libm's double
+ * exp over a sweep. */
+float orc_expf(float x)
+{
+  if (x != x) return x;
+  if (x > 89.f) return INFINITY;
+  if (x < -104.f) return 0.f;
+  const float k = rintf(x * 0x1.715476p+0f);
+  float       r = fmaf(-k, 0x1.62e430p-1f, x);
+  r = fmaf(-k, -0x1.05c610p-29f, r);
+  float p = fmaf(r, 1.f / 5040.f, 1.f / 720.f);
+  p = fmaf(r, p, 1.f / 120.f);
+  p = fmaf(r, p, 1.f / 24.f);
+  p = fmaf(r, p, 1.f / 6.f);
+  p = fmaf(r, p, 0.5f);
+  p = fmaf(r, p, 1.f);
+  p = fmaf(r, p, 1.f);
+  const int ki = (int)k, k1 = ki / 2, k2 = ki - k1;
+  return (p * orc_pow2i(k1)) * orc_pow2i(k2);
+}
+
 /* OpenCL fract(): min(x - floor(x), 0x1.fffffep-1f). */
 float orc_fractf(float x)
 {
@@ -695,6 +732,7 @@ static double orc_sum_f32fix(const float *a, int nx, int ny)
 #define R_MAX fmaxf
 #define R_MIN fminf
 #define R_HYPOT orc_hypotf
+#define R_EXP orc_expf
 #define R_POW orc_powf
 #define R_VOLUME orc_sum_f32fix
 #define R_STORE orc_store_f32
@@ -710,6 +748,7 @@ static double orc_sum_f32fix(const float *a, int nx, int ny)
 #undef R_MAX
 #undef R_MIN
 #undef R_HYPOT
+#undef R_EXP
 #undef R_POW
 #undef R_VOLUME
 #undef R_STORE
@@ -725,6 +764,7 @@ static double orc_sum_f32fix(const float *a, int nx, int ny)
 #define R_MAX fmax
 #define R_MIN fmin
 #define R_HYPOT orc_hypotd
+#define R_EXP exp
 #define R_POW orc_powd
 #define R_VOLUME orc_sum_f64
 #define R_STORE orc_store_f64

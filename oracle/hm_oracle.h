@@ -222,6 +222,34 @@ void orc_flow_simulation_f64(const double *z, int nx, int ny, float water_height
                              int flux_diffusion, float flux_diffusion_strength,
                              float dry_out_ratio, double *d);
 
+/* exp of the FP32 contract (shared with the kernels). */
+float orc_expf(float x);
+
+/* hmap::gpu::snow_simulation: the final snow depth into s (nx*ny). */
+void orc_snow_simulation_f32(const float *z, int nx, int ny, float snow_depth,
+                             const float *fall_map, const float *melting_map,
+                             const float *talus, int iterations, float dt,
+                             float fall_iterations_ratio, float k_snow, float k_visc,
+                             float k_melt_factor, float k_depth_ratio,
+                             float k_depth_slope_ratio, int post_filter,
+                             float thermal_talus_ratio, float *s);
+void orc_snow_simulation_f64(const double *z, int nx, int ny, float snow_depth,
+                             const double *fall_map, const double *melting_map,
+                             const double *talus, int iterations, float dt,
+                             float fall_iterations_ratio, float k_snow, float k_visc,
+                             float k_melt_factor, float k_depth_ratio,
+                             float k_depth_slope_ratio, int post_filter,
+                             float thermal_talus_ratio, double *s);
+/* One snow_simulation kernel launch (s -> so, Jacobi), k_melt as passed. */
+void orc_snow_step_f32(const float *z, const float *s, const float *talus,
+                       const float *melt, float *so, int nx, int ny, float dt,
+                       float snow_depth, float k_snow, float k_melt, float k_visc,
+                       float k_depth_ratio, float k_depth_slope_ratio);
+void orc_snow_step_f64(const double *z, const double *s, const double *talus,
+                       const double *melt, double *so, int nx, int ny, double dt,
+                       double snow_depth, double k_snow, double k_melt, double k_visc,
+                       double k_depth_ratio, double k_depth_slope_ratio);
+
 /* Single-pass entry points, for pass-level tests. */
 void orc_vp_flow_f32(const float *z, const float *d1, const float *fl,
                      const float *fr, const float *ft, const float *fb,

@@ -145,6 +145,15 @@ def lib():
                                           fl, fl, F32P]
     L.orc_flow_simulation_f64.argtypes = [F64P, C.c_int, C.c_int, fl, F64P, C.c_int, fl, C.c_int,
                                           fl, fl, F64P]
+    L.orc_expf.argtypes = [fl]
+    L.orc_expf.restype = fl
+    snow_tail = [C.c_int] + [fl] * 7 + [C.c_int, fl]
+    L.orc_snow_simulation_f32.argtypes = ([F32P, C.c_int, C.c_int, fl, F32P, F32P, F32P]
+                                          + snow_tail + [F32P])
+    L.orc_snow_simulation_f64.argtypes = ([F64P, C.c_int, C.c_int, fl, F64P, F64P, F64P]
+                                          + snow_tail + [F64P])
+    L.orc_snow_step_f32.argtypes = [F32P] * 5 + [C.c_int, C.c_int] + [fl] * 7
+    L.orc_snow_step_f64.argtypes = [F64P] * 5 + [C.c_int, C.c_int] + [C.c_double] * 7
     _lib = L
     return L
 
@@ -349,3 +358,56 @@ def flow_simulation(z, water_height, depth_map, iterations, dt=0.5, flux_diffusi
     fn(z, nx, ny, float(water_height), dm, int(iterations), float(dt), int(bool(flux_diffusion)),
        float(flux_diffusion_strength), float(dry_out_ratio), d)
     return d
+
+
+# ------------------------------------------------------------------ snow
+
+SNOW_DEFAULTS = dict(dt=0.5, fall_iterations_ratio=1.0, k_snow=0.5, k_visc=0.1, k_melt_factor=0.8,
+                     k_depth_ratio=1.0, k_depth_slope_ratio=1.0, post_filter=True,
+                     thermal_talus_ratio=0.2)
+
+
+def expf(x):
+    """The FP32 contract's exp (shared with the kernels)."""
+    return float(lib().orc_expf(float(x)))
+
+
+def snow_simulation(z, snow_depth, fall_map, melting_map, talus, iterations, dtype=np.float32,
+                    **kw):
+    """hmap::gpu::snow_simulation restated; returns the snow depth.  talus,
+    fall_map and melting_map are planes of z's shape (scalars broadcast)."""
+    p = dict(SNOW_DEFAULTS)
+    unknown = set(kw) - set(p)
+    if unknown:
+        raise TypeError("unknown parameters %s" % sorted(unknown))
+    p.update(kw)
+    z = np.ascontiguousarray(z, dtype)
+    ny, nx = z.shape
+
+    def plane(a):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype), z.shape), dtype)
+
+    f, m, t = plane(fall_map), plane(melting_map), plane(talus)
+    s = np.empty_like(z)
+    fn = lib().orc_snow_simulation_f32 if dtype == np.float32 else lib().orc_snow_simulation_f64
+    fn(z, nx, ny, float(snow_depth), f, m, t, int(iterations), p["dt"],
+       p["fall_iterations_ratio"], p["k_snow"], p["k_visc"], p["k_melt_factor"],
+       p["k_depth_ratio"], p["k_depth_slope_ratio"], int(bool(p["post_filter"])),
+       p["thermal_talus_ratio"], s)
+    return s
+
+
+def snow_step(z, s, talus, melt, dt=0.5, snow_depth=0.1, k_snow=0.5, k_melt=0.0, k_visc=0.1,
+              k_depth_ratio=1.0, k_depth_slope_ratio=1.0, dtype=np.float32):
+    """One snow_simulation kernel launch (no fall, no border fill)."""
+    z = np.ascontiguousarray(z, dtype)
+    ny, nx = z.shape
+
+    def plane(a):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype), z.shape), dtype)
+
+    so = np.empty_like(z)
+    fn = lib().orc_snow_step_f32 if dtype == np.float32 else lib().orc_snow_step_f64
+    fn(z, plane(s), plane(talus), plane(melt), so, nx, ny, dt, snow_depth, k_snow, k_melt, k_visc,
+       k_depth_ratio, k_depth_slope_ratio)
+    return so

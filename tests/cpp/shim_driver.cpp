@@ -3,7 +3,8 @@
 // examples/ex_hydraulic_vpipes_gpu/ex_hydraulic_vpipes_gpu.cpp:7-22,
 // examples/ex_flow_simulation/ex_flow_simulation.cpp:5-26,
 // examples/ex_thermal_auto_bedrock/ex_thermal_auto_bedrock.cpp:5-15,
-// examples/ex_sediment_deposition/ex_sediment_deposition.cpp:5-18) and
+// examples/ex_sediment_deposition/ex_sediment_deposition.cpp:5-18,
+// examples/ex_snow_simulation/ex_snow_simulation.cpp:5-38) and
 // writes every result as raw float32 into the directory given, for the GPU
 // test to compare with the oracle.
 #include <cstdio>
@@ -96,6 +97,22 @@ int main(int argc, char **argv)
   hmap::gpu::sediment_deposition(zs, talus_s, &dep_s, 0.01f, 10, 50);
   dump(dir, "sed256", zs);
   dump(dir, "sed_dep256", dep_s);
+
+  // snow_simulation, as ex_snow_simulation (GPU noise in place of its CPU
+  // noise and bulkify; a ramp melting map in place of snow_melting_map)
+  hmap::Array zw = hmap::gpu::noise_fbm(hmap::NoiseType::PERLIN, shape, {4.f, 4.f}, 2);
+  hmap::remap(zw);
+  dump(dir, "snow_in", zw);
+  const hmap::Array talus_w(shape, 2.5f / shape.x);
+  hmap::Array snow0 = hmap::gpu::snow_simulation(zw, 1e-1f, hmap::Array(shape, 1.f),
+                                                 hmap::Array(shape, 0.f), talus_w, 500);
+  dump(dir, "snow0", snow0);
+  hmap::Array melt(shape);
+  for (int j = 0; j < shape.y; j++)
+    for (int i = 0; i < shape.x; i++) melt(i, j) = (float)j / (float)(shape.y - 1);
+  hmap::Array snow1 = hmap::gpu::snow_simulation(zw, 1e-1f, hmap::Array(shape, 1.f), melt,
+                                                 talus_w, 500);
+  dump(dir, "snow1", snow1);
   std::printf("shim_driver ok\n");
   return 0;
 }

@@ -79,3 +79,14 @@ def test_shim_sediment_deposition(out, orc):
     z, dep = orc.sediment_deposition(z0, talus, 0.01, 10, 50)
     assert np.array_equal(out("sed256", (256, 256)), z)
     assert np.array_equal(out("sed_dep256", (256, 256)), dep)
+
+
+def test_shim_snow_simulation(out, orc):
+    zin = out("snow_in", (256, 256))
+    assert np.array_equal(zin, orc.remap(orc.noise_fbm(256, 256, (4.0, 4.0), 2)))
+    talus = np.float32(2.5) / np.float32(256)
+    ref0 = orc.snow_simulation(zin, 0.1, 1.0, 0.0, talus, 500)
+    assert np.array_equal(out("snow0", (256, 256)), ref0)
+    melt = np.repeat((np.arange(256, dtype=np.float32) / np.float32(255))[:, None], 256, axis=1)
+    ref1 = orc.snow_simulation(zin, 0.1, 1.0, melt, talus, 500)
+    assert np.array_equal(out("snow1", (256, 256)), ref1)
