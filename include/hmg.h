@@ -1,8 +1,8 @@
 /*
  * hmg.h -- C ABI of libhmg, the MI355X (gfx950) implementation of HighMap's
  * grid-erosion hot path: gpu::noise_fbm / gpu::noise, gpu::thermal (and
- * thermal_auto_bedrock, sediment_deposition), gpu::hydraulic_vpipes and
- * gpu::flow_simulation.
+ * thermal_auto_bedrock, sediment_deposition, snow_simulation),
+ * gpu::hydraulic_vpipes and gpu::flow_simulation.
  *
  * This header replaces the OpenCL runtime layer the reference binds through
  * (HighMap/src/gpu_opencl/gpu_opencl.cpp and the CLWrapper `Run` calls in
