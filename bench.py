@@ -9,7 +9,7 @@ k_capacity 0.1, k_erode 0.05, k_depose 0.05, evaporation 0.01).  The default
 50 timed steps are one full 500-iteration job.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--storage f32|bf16|f16]
-    python bench.py --workload batch|serve|precision|thermal256|vpipes1024|flow|snow|copy
+    python bench.py --workload batch|serve|precision|thermal256|vpipes1024|flow|snow|tiles|copy
 
 Each run prints one JSON line.  For the headline, `value` is hydraulic
 cell-iterations per second (nx*ny*iterations / wall time of the timed steps,
@@ -526,6 +526,27 @@ def run_snow(args):
                              "sample": "the example configuration, 256^2 x 500, median of 3"}}
 
 
+def run_tiles(args):
+    """hydraulic_vpipes run tile by tile over a hmap::VirtualArray (C++
+    drop-in, highmap_amd/va_bench): a 16384^2 grid (1 GiB of float32) held
+    on the host as 4096^2 tiles with 64-cell halos, 50 iterations per tile,
+    each tile uploaded, eroded and downloaded by the drop-in call; the
+    distributed mode (one host thread per core, one device context per
+    thread up to 4) against the sequential one.  PCIe-inclusive rates."""
+    exe = os.path.join(ROOT, "highmap_amd", "va_bench")
+    if not os.path.exists(exe):
+        raise RuntimeError("highmap_amd/va_bench is not built (python -m highmap_amd.build)")
+    r = subprocess.run([exe, "16384", "4096", "64", "50"], capture_output=True, text=True,
+                       timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError("va_bench failed: " + r.stdout + r.stderr)
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"metric": "Gcell-iter/s vpipes over a 16384^2 VirtualArray (4096^2 tiles)",
+            "unit": "Gcell-iter/s", "n_gpus": 1, "value": res["dist_gcell_iter_s"],
+            "data": "synthetic: noise_fbm PERLIN per tile over its bbox, remapped",
+            "detail": res}
+
+
 def run_vpipes1024(args):
     """hydraulic_vpipes at 1024^2 (seed 2), 200 iterations: (i) reference
     semantics (gravity 1, volume top-up) with plain and fused launches,
@@ -609,7 +630,7 @@ def main():
                     help="serve: cells of the jobs that run at once (0: no cap)")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batch", "serve", "precision", "thermal256",
-                             "vpipes1024", "flow", "snow", "copy"])
+                             "vpipes1024", "flow", "snow", "tiles", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -621,7 +642,7 @@ def main():
         sys.exit(2)
     fn = {"headline": run_headline, "batch": run_batch, "precision": run_precision,
           "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "flow": run_flow,
-          "snow": run_snow, "copy": run_copy}
+          "snow": run_snow, "tiles": run_tiles, "copy": run_copy}
     if args.workload == "serve":
         from highmap_amd import serve
 

@@ -78,15 +78,21 @@ def build(verbose=False):
     shim = os.path.join(CSRC, "hmap_gpu.cpp")
     if os.path.exists(shim):
         tmp = SHIM_LIB + ".tmp"
-        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
-              "-I" + INC, shim, "-o", tmp, "-L" + HERE, "-lhmg",
+        va = os.path.join(CSRC, "virtual_array.cpp")
+        _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
+              "-I" + INC, shim, va, "-o", tmp, "-L" + HERE, "-lhmg",
               "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, SHIM_LIB)
-        for name in ("shim_driver", "io_driver"):
+        bench = os.path.join(HERE, "va_bench")
+        _run(["g++", "-std=c++20", "-O2", "-pthread", "-I" + INC,
+              os.path.join(CSRC, "va_bench.cpp"), "-o", bench + ".tmp", "-L" + HERE,
+              "-lhmap_gpu", "-lhmg", "-Wl,-rpath,$ORIGIN"])
+        os.replace(bench + ".tmp", bench)
+        for name in ("shim_driver", "io_driver", "va_driver"):
             drv = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
             if os.path.exists(drv):
                 out = os.path.join(ROOT, "tests", "cpp", name)
-                _run(["g++", "-std=c++17", "-O2", "-I" + INC, drv, "-o", out + ".tmp",
+                _run(["g++", "-std=c++20", "-O2", "-pthread", "-I" + INC, drv, "-o", out + ".tmp",
                       "-L" + HERE, "-lhmap_gpu", "-lhmg",
                       "-Wl,-rpath,$ORIGIN/../../highmap_amd"])
                 os.replace(out + ".tmp", out)

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -191,15 +192,51 @@ namespace gpu
 
 namespace
 {
-std::mutex g_mu;
-hmg_ctx   *g_ctx = nullptr;
+// Calls from different host threads run on different libhmg contexts, each
+// with its own HIP stream and buffer pool (at most HMAP_GPU_CONTEXTS of
+// them, default 4; a thread keeps the one it first gets), so that tiles
+// processed on several threads (VirtualArray's VA_DISTRIBUTED mode) overlap
+// their uploads, kernels and downloads on the device instead of queueing on
+// one context's lock.  Results do not depend on the context.
+std::mutex              g_mu;
+std::vector<hmg_ctx *>  g_ctxs;
+size_t                  g_next = 0;
+thread_local hmg_ctx   *t_ctx = nullptr;
+
+size_t max_contexts()
+{
+  const char *e = std::getenv("HMAP_GPU_CONTEXTS");
+  const long  v = e ? std::strtol(e, nullptr, 10) : 4;
+  return (size_t)std::max(1L, std::min(16L, v));
+}
+
+hmg_ctx *first_ctx_locked()
+{
+  if (g_ctxs.empty())
+  {
+    hmg_ctx *c = nullptr;
+    if (hmg_init(0, &c) != HMG_OK)
+      throw std::runtime_error(std::string("hmap::gpu: no GPU: ") + hmg_last_error());
+    g_ctxs.push_back(c);
+  }
+  return g_ctxs.front();
+}
 
 hmg_ctx *ctx()
 {
+  if (t_ctx) return t_ctx;
   std::lock_guard<std::mutex> lk(g_mu);
-  if (!g_ctx && hmg_init(0, &g_ctx) != HMG_OK)
-    throw std::runtime_error(std::string("hmap::gpu: no GPU: ") + hmg_last_error());
-  return g_ctx;
+  first_ctx_locked();
+  const size_t k = g_next++ % max_contexts();
+  if (k >= g_ctxs.size())
+  {
+    hmg_ctx *c = nullptr;
+    if (hmg_init(0, &c) != HMG_OK)
+      throw std::runtime_error(std::string("hmap::gpu: context: ") + hmg_last_error());
+    g_ctxs.push_back(c);
+  }
+  t_ctx = g_ctxs[std::min(k, g_ctxs.size() - 1)];
+  return t_ctx;
 }
 
 void check(int rc, const char *what)
@@ -218,8 +255,15 @@ void check_shape(const Array &a, glm::ivec2 s, const char *what)
 bool init_opencl()
 {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (g_ctx) return true;
-  return hmg_init(0, &g_ctx) == HMG_OK;
+  try
+  {
+    first_ctx_locked();
+  }
+  catch (const std::runtime_error &)
+  {
+    return false;
+  }
+  return true;
 }
 
 // HighMap/src/primitives/coherent_noise/primitives_gpu.cpp:504-541

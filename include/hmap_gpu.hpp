@@ -49,6 +49,14 @@ struct vec4
   vec4() = default;
   vec4(float a, float b, float c, float d) : x(a), y(b), z(c), w(d) {}
 };
+struct ivec4
+{
+  int x = 0, y = 0, z = 0, w = 0;
+  ivec4() = default;
+  ivec4(int a, int b, int c, int d) : x(a), y(b), z(c), w(d) {}
+};
+inline bool operator==(const ivec2 &a, const ivec2 &b) { return a.x == b.x && a.y == b.y; }
+inline bool operator!=(const ivec2 &a, const ivec2 &b) { return !(a == b); }
 } // namespace glm
 #endif
 
