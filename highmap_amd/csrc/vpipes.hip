@@ -276,6 +276,9 @@ __global__ void __launch_bounds__(256) k_vp_sum(const typename S::T *__restrict_
 // Above and below, loads use clamped rows, and the computed rows B/C(-1)
 // and B/C(ny) are replaced by those of row 0 and ny-1.
 
+#ifndef HMG_GA_ATTR
+#define HMG_GA_ATTR
+#endif
 constexpr int STRIP_OUT = 60; // columns written per wavefront
 constexpr int WPB = 4;        // wavefronts (adjacent strips) per workgroup
 
@@ -371,7 +374,7 @@ __device__ __forceinline__ int xcd_wave_item()
 // FLOW: the flow and water passes only (flow_simulation): d1 is the stored
 // depth, no transport, erosion or volume sum; only D and F are written
 // (Sp, rain, zo, So and the slot sets are not touched).
-template <class S, bool URAIN, bool PF, bool KEXP1, bool FLOW = false>
+template <class S, bool URAIN, bool PF, bool KEXP1, bool FLOW = false, bool GA = false>
 __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
                                                const typename S::T *__restrict__ D,
                                                const Flux4<S> *__restrict__ F,
@@ -450,11 +453,16 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   // through the slots by renaming instead of register moves.  Row r + q of
   // an unrolled pass finds A(r+q-1) in slot q, A(r+q) in q+1, ..., and
   // B/C(r+q-1), B/C(r+q), B/C(r+q+1) in slots q, q+1, q+2 (mod U).  With
-  // the load-ahead row, five A rows are live at once.
-  constexpr int  U = PF ? 5 : 4;
+  // the load-ahead row, five A rows are live at once.  GA (gathers ahead):
+  // rows are loaded two rows ahead and each row's sediment gathers are
+  // issued one row before they are consumed, so every load has a whole row
+  // step to arrive; six A rows and one row of gathered texels are live.
+  static_assert(!GA || (PF && !FLOW), "gathers ahead imply the load-ahead row");
+  constexpr int  U = GA ? 6 : (PF ? 5 : 4);
   constexpr bool ROW_SYNC = !std::is_same<S, StBF16>::value;
-  RowA  R[U];
-  float Sr[U], Cr[U][4];
+  RowA       R[U];
+  float      Sr[U], Cr[U][4];
+  TexelFetch TFr[2]; // GA: gathers of the next row to finish, by parity
 
   // pre-roll: A(j0-2..j0+1), B/C(j0-1), B/C(j0)
   {
@@ -467,6 +475,11 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
     if (!FLOW) Sr[1] = transport(R[0], R[1], R[2], j0);
     flow(R[0], R[1], R[2], Cr[1]);
     if (PF) R[3] = load(j0 + 2);
+    if (GA)
+    {
+      TFr[0] = transport_issue(R[1], R[2], R[3], j0 + 1);
+      R[4] = load(j0 + 3);
+    }
   }
 
   Fix128 acc{0, 0};
@@ -524,15 +537,31 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
       // against the scheduler: loads cannot cross a memory clobber, the flow
       // starts from values redefined after the loads, and the gathered
       // samples are first used after the flow results.
-      TexelFetch tf = transport_issue(a0, ap1, ap2, r + 1);
-      asm volatile("" ::: "memory");
-      if (PF) R[(q + 4) % U] = load(r + 3);
-      asm volatile("" : "+v"(ap1.z), "+v"(ap1.d1)::"memory");
-      flow(a0, ap1, ap2, c_p1);
-      asm volatile(""
-                   : "+v"(c_p1[0]), "+v"(c_p1[1]), "+v"(c_p1[2]), "+v"(c_p1[3]), "+v"(tf.s00),
-                     "+v"(tf.s10), "+v"(tf.s01), "+v"(tf.s11));
-      s_p1 = transport_finish(tf);
+      if (GA)
+      {
+        // gathers of row r+2 (rows r+1..r+3 are in), then the row r+4
+        // loads, then the flow of row r+1; the gathers of row r+1 were
+        // issued one step ago
+        RowA &ap3 = R[(q + 4) % U];
+        TFr[(q + 1) % 2] = transport_issue(ap1, ap2, ap3, r + 2);
+        asm volatile("" ::: "memory");
+        R[(q + 5) % U] = load(r + 4);
+        asm volatile("" : "+v"(ap1.z), "+v"(ap1.d1)::"memory");
+        flow(a0, ap1, ap2, c_p1);
+        s_p1 = transport_finish(TFr[q % 2]);
+      }
+      else
+      {
+        TexelFetch tf = transport_issue(a0, ap1, ap2, r + 1);
+        asm volatile("" ::: "memory");
+        if (PF) R[(q + 4) % U] = load(r + 3);
+        asm volatile("" : "+v"(ap1.z), "+v"(ap1.d1)::"memory");
+        flow(a0, ap1, ap2, c_p1);
+        asm volatile(""
+                     : "+v"(c_p1[0]), "+v"(c_p1[1]), "+v"(c_p1[2]), "+v"(c_p1[3]), "+v"(tf.s00),
+                       "+v"(tf.s10), "+v"(tf.s01), "+v"(tf.s11));
+        s_p1 = transport_finish(tf);
+      }
 
       // row r: water pass and erosion
       const bool top = r == 0, bot = r == ny - 1;
@@ -584,7 +613,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
 }
 
 // One iteration of one session.
-template <class S, bool URAIN, bool PF, bool KEXP1>
+template <class S, bool URAIN, bool PF, bool KEXP1, bool GA = false>
 __global__ void __launch_bounds__(64 * WPB) k_vp_stream(const float *__restrict__ z,
                                                    const typename S::T *__restrict__ D,
                                                    const Flux4<S> *__restrict__ F,
@@ -605,8 +634,35 @@ __global__ void __launch_bounds__(64 * WPB) k_vp_stream(const float *__restrict_
 {
   const int item = xcd_wave_item();
   if (item >= n_items) return;
-  vp_stream_item<S, URAIN, PF, KEXP1>(z, D, F, Sp, rain, zo, Do, Fo, So, prev_set, set,
-                                      clear_set, item, strips, chunk_rows, k, y);
+  vp_stream_item<S, URAIN, PF, KEXP1, false, GA>(z, D, F, Sp, rain, zo, Do, Fo, So, prev_set, set,
+                                                 clear_set, item, strips, chunk_rows, k, y);
+}
+
+// The gathers-ahead form (see vp_stream_item).
+template <class S, bool URAIN, bool KEXP1>
+__global__ void __launch_bounds__(64 * WPB) HMG_GA_ATTR k_vp_stream_ga(const float *__restrict__ z,
+                                                           const typename S::T *__restrict__ D,
+                                                           const Flux4<S> *__restrict__ F,
+                                                           const typename S::T *__restrict__ Sp,
+                                                           const float *__restrict__ rain,
+                                                           float *__restrict__ zo,
+                                                           typename S::T *__restrict__ Do,
+                                                           Flux4<S> *__restrict__ Fo,
+                                                           typename S::T *__restrict__ So,
+                                                           const Fix128 *__restrict__ prev_set,
+                                                           Fix128 *set,
+                                                           Fix128 *clear_set,
+                                                           int     strips,
+                                                           int     chunk_rows,
+                                                           int     n_items,
+                                                           VpConst k,
+                                                           VpDyn   y)
+{
+  const int item = xcd_wave_item();
+  if (item >= n_items) return;
+  vp_stream_item<S, URAIN, true, KEXP1, false, true>(z, D, F, Sp, rain, zo, Do, Fo, So, prev_set,
+                                                     set, clear_set, item, strips, chunk_rows, k,
+                                                     y);
 }
 
 // ------------------------------------------------------------ batched form
@@ -885,8 +941,25 @@ static int session_step_fused(hmg_session *s)
 #define HMG_STREAM(UR, PFV, KE)                                                \
   k_vp_stream<S, UR, PFV, KE><<<G, 64 * WPB, 0, st>>>(z, D, F, Sp, rain, zo, Do, Fo, So, pset, \
                                                  aset, cset, strips, ch, n_items, s->k, y)
+#define HMG_STREAM_GA(UR, KE)                                                  \
+  k_vp_stream_ga<S, UR, KE><<<G, 64 * WPB, 0, st>>>(z, D, F, Sp, rain, zo, Do, Fo, So, pset,     \
+                                                  aset, cset, strips, ch, n_items, s->k, y)
   const bool ke1 = s->k.kexp == 1.f;
-  if (rain)
+  if (s->prefetch == 2)
+  {
+    if (rain)
+    {
+      if (ke1)
+        HMG_STREAM_GA(false, true);
+      else
+        HMG_STREAM_GA(false, false);
+    }
+    else if (ke1)
+      HMG_STREAM_GA(true, true);
+    else
+      HMG_STREAM_GA(true, false);
+  }
+  else if (rain)
   {
     if (ke1)
       HMG_STREAM(false, true, true);
@@ -908,6 +981,7 @@ static int session_step_fused(hmg_session *s)
       HMG_STREAM(true, true, false);
   }
 #undef HMG_STREAM
+#undef HMG_STREAM_GA
   HMG_TRY(check_launch("vp_stream"));
   s->cur = b;
   s->iter++;
@@ -970,8 +1044,10 @@ struct TuneKey
 };
 std::mutex                                   g_shape_mu;
 std::map<TuneKey, std::pair<int, int>>       g_shape_cache;
-constexpr int                                TUNE_CAND[][2] = {{24, 1}, {32, 1}, {40, 1}, {47, 1},
-                                                               {56, 1}, {64, 1}, {47, 0}, {56, 0}};
+// (rows per wavefront, variant): 0 rows loaded when needed, 1 one row
+// ahead, 2 two rows ahead with the gathers one row ahead
+constexpr int TUNE_CAND[][2] = {{24, 1}, {32, 1}, {40, 1}, {47, 1}, {56, 1}, {64, 1},
+                                {47, 0}, {56, 0}, {47, 2}, {72, 2}, {94, 2}};
 constexpr int                                N_CAND = sizeof(TUNE_CAND) / sizeof(TUNE_CAND[0]);
 constexpr int                                N_PROBE = 1 + 2 * N_CAND; // warm-up + 2 rounds
 
@@ -1327,7 +1403,8 @@ int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch)
   if (!s) return set_error(HMG_ERR_INVALID, "null session");
   if (chunk_rows < 1 || chunk_rows > 65536)
     return set_error(HMG_ERR_INVALID, "chunk_rows must be in [1, 65536]");
-  if (prefetch != 0 && prefetch != 1) return set_error(HMG_ERR_INVALID, "prefetch must be 0 or 1");
+  if (prefetch < 0 || prefetch > 2)
+    return set_error(HMG_ERR_INVALID, "prefetch must be 0, 1 or 2");
   std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
   tune_stop(s); // the caller's shape wins over online tuning
   s->chunk_rows = chunk_rows;

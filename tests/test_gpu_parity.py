@@ -225,7 +225,7 @@ def test_session_16bit_storage_bitexact(hm, orc, mode, storage):
 def test_fused_launch_shapes_do_not_change_results(hm, orc):
     z0 = _terrain(orc, 300, 170, 13)
     outs = []
-    for ch, pf in ((1, 0), (7, 1), (32, 0), (64, 1), (200, 1)):
+    for ch, pf in ((1, 0), (7, 1), (32, 0), (64, 1), (200, 1), (1, 2), (5, 2), (72, 2), (200, 2)):
         s = hm.Session(z0, **VP_WORKLOAD)
         s.set_tuning(ch, pf)
         s.vpipes(9)
@@ -233,6 +233,26 @@ def test_fused_launch_shapes_do_not_change_results(hm, orc):
         s.close()
     for o in outs[1:]:
         _assert_same(o, outs[0])
+
+
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_gathers_ahead_variant_matches_oracle(hm, orc, storage):
+    """The gathers-ahead form of the fused kernel (launch variant 2) with a
+    rain plane, thermal steps between calls and 16-bit storage: the same
+    state as the oracle."""
+    nx, ny = 203, 141
+    z0 = _terrain(orc, nx, ny, 21)
+    rain = np.random.default_rng(4).random((ny, nx), dtype=np.float32)
+    kw = dict(VP_WORKLOAD, rain_per_iter=1e-4)
+    s = hm.Session(z0, rain_map=rain, storage=storage, **kw)
+    s.set_tuning(17, 2)
+    o = orc.VpState(z0, orc.vp_params(storage=storage, **kw), rain=rain)
+    for _ in range(2):
+        s.vpipes(6).thermal(1.0 / nx, 1)
+        o.run(6).thermal_step(1.0 / nx, 1)
+    g = s.export()
+    s.close()
+    _assert_same(g, (o.z, o.d, o.s, o.u, o.v))
 
 
 @pytest.mark.parametrize("storage", [0, 1])
