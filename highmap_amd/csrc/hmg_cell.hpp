@@ -33,7 +33,25 @@ struct StBF16
   {
     return __uint_as_float(((uint32_t)x) << 16);
   }
+  // Round to nearest even.  gfx950's conversion instruction
+  // (v_cvt_pk_bf16_f32) gives exactly that for every non-NaN float,
+  // subnormals and infinities included (compared with the integer form
+  // below over all 2^32 patterns); NaNs keep the integer form's rule (the
+  // top 16 bits, quiet bit set when low payload bits are dropped), taken in
+  // a branch that no lane enters unless it holds a NaN.
   static __device__ __forceinline__ T st(float x)
+  {
+    const __bf16 h = (__bf16)x;
+    T            r = __builtin_bit_cast(T, h);
+    if (__builtin_expect(x != x, 0))
+    {
+      const uint32_t b = __float_as_uint(x);
+      r = (T)((b >> 16) | ((b & 0xffffu) ? 0x40u : 0u));
+    }
+    return r;
+  }
+  // the integer form: add half an ulp (ties to even) and truncate
+  static __device__ __forceinline__ T st_int(float x)
   {
     uint32_t b = __float_as_uint(x);
     if ((b & 0x7f800000u) == 0x7f800000u) return (T)((b >> 16) | ((b & 0xffffu) ? 0x40u : 0u));

@@ -24,7 +24,7 @@ def test_snow_example_configuration(hm, orc):
     assert got.max() > 0
 
 
-@pytest.mark.parametrize("shape", [(3, 3), (5, 4), (96, 64), (130, 257)])
+@pytest.mark.parametrize("shape", [(3, 3), (5, 4), (63, 5), (125, 9), (96, 64), (130, 257)])
 @pytest.mark.parametrize("iters,kw", [
     (0, {}), (1, {}), (2, dict(post_filter=False)),
     (37, dict(fall_iterations_ratio=0.4, post_filter=False)),
