@@ -351,6 +351,81 @@ glm::vec2 VirtualArray::range(const ComputeMode &cm) const
                    *std::max_element(hi.begin(), hi.end()));
 }
 
+glm::vec2 VirtualArray::range_percentile(float p_low, float p_high, const ComputeMode &cm,
+                                         size_t bins) const
+{
+  // virtual_array_processing.cpp:96-164: a histogram of the interior cells
+  // over the whole-tile range, then the first bins reaching p_low and
+  // p_high of the count
+  const glm::ivec2 nt = get_max_tiles();
+  if (nt.x * nt.y == 0) return glm::vec2(0.f, 0.f);
+  const glm::vec2 rg = range(cm);
+  if (rg.x == rg.y) return rg;
+  std::vector<std::vector<size_t>> part(nt.x * nt.y, std::vector<size_t>(bins, 0));
+  for_each_tile(
+      *this,
+      [&](const Array &tile, const TileRegion &region)
+      {
+        auto             &hist = part[region.key.ty * nt.x + region.key.tx];
+        const glm::ivec4 &h = region.halo;
+        for (int j = h.z; j < region.shape.y - h.w; ++j)
+          for (int i = h.x; i < region.shape.x - h.y; ++i)
+          {
+            const float v = tile(i, j);
+            hist[std::min<size_t>(bins - 1, size_t((v - rg.x) / (rg.y - rg.x) * bins))]++;
+          }
+      },
+      cm);
+  std::vector<size_t> hist(bins, 0);
+  for (const auto &ph : part)
+    for (size_t b = 0; b < bins; ++b) hist[b] += ph[b];
+  size_t total = 0;
+  for (size_t v : hist) total += v;
+  const size_t lo_target = size_t(p_low * total), hi_target = size_t(p_high * total);
+  size_t       acc = 0;
+  float        lo = rg.x, hi = rg.y;
+  for (size_t b = 0; b < bins; ++b)
+  {
+    acc += hist[b];
+    if (acc >= lo_target && lo == rg.x) lo = rg.x + (rg.y - rg.x) * (float(b) / bins);
+    if (acc >= hi_target)
+    {
+      hi = rg.x + (rg.y - rg.x) * (float(b) / bins);
+      break;
+    }
+  }
+  return glm::vec2(lo, hi);
+}
+
+std::vector<float> VirtualArray::unique_values(const ComputeMode &cm) const
+{
+  // virtual_array_processing.cpp:253-281 and vector_unique_values
+  // (vector_utils.cpp:280-287): the sorted distinct values of every tile,
+  // halo cells included
+  const glm::ivec2 nt = get_max_tiles();
+  if (nt.x * nt.y == 0) return {0.f};
+  auto uniq = [](std::vector<float> &v)
+  {
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  std::vector<std::vector<float>> part(nt.x * nt.y);
+  for_each_tile(
+      *this,
+      [&](const Array &tile, const TileRegion &region)
+      {
+        std::vector<float> v = tile.vector;
+        uniq(v);
+        part[region.key.ty * nt.x + region.key.tx] = std::move(v);
+      },
+      cm);
+  std::vector<float> all;
+  for (const auto &v : part) all.insert(all.end(), v.begin(), v.end());
+  uniq(all);
+  return all;
+}
+
 float VirtualArray::sum(const ComputeMode &cm) const
 {
   const glm::ivec2 nt = get_max_tiles();

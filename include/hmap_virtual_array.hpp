@@ -12,9 +12,8 @@
 //
 // Provided: RAM tile storage; the VA_SEQUENTIAL, VA_DISTRIBUTED and
 // VA_SINGLE_ARRAY (stride 1) compute modes.  Not provided (they throw
-// std::invalid_argument): the disk storages, strided and downscaled
-// single-array modes (they need HighMap's resampling), range_percentile and
-// unique_values.
+// std::invalid_argument): the disk storages and the strided and downscaled
+// single-array modes (they need HighMap's resampling).
 //
 // Tiles of one for_each_tile call in VA_DISTRIBUTED mode run on separate
 // host threads; the hmap::gpu functions give each calling thread its own
@@ -165,6 +164,9 @@ struct VirtualArray
   float     min(const ComputeMode &cm) const;
   float     sum(const ComputeMode &cm) const;
   glm::vec2 range(const ComputeMode &cm) const;
+  glm::vec2 range_percentile(float p_low, float p_high, const ComputeMode &cm,
+                             size_t bins = 1024) const;
+  std::vector<float> unique_values(const ComputeMode &cm) const;
 
   void inverse(const ComputeMode &cm);
   void remap(float vmin, float vmax, const ComputeMode &cm);

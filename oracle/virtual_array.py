@@ -170,6 +170,37 @@ class VA:
             offset = F(vmin - F(lo * scale))
             t[...] = (t * scale).astype(np.float32) + offset
 
+    def range_percentile(self, p_low, p_high, bins=1024):
+        """virtual_array_processing.cpp:96-164."""
+        lo_r, hi_r = self.min(), self.max()
+        if lo_r == hi_r:
+            return lo_r, hi_r
+        hist = np.zeros(bins, np.int64)
+        span = F(hi_r - lo_r)
+        for r in self.regions():
+            hl, hr, hb, ht = r.halo
+            v = self.tile(r)[hb:r.shape[1] - ht, hl:r.shape[0] - hr].ravel()
+            idx = (((v - lo_r) / span).astype(np.float32) * F(bins)).astype(np.float32)
+            idx = np.minimum(idx.astype(np.int64), bins - 1)
+            hist += np.bincount(idx, minlength=bins)
+        total = int(hist.sum())
+        lo_t = int(F(F(p_low) * F(total)))
+        hi_t = int(F(F(p_high) * F(total)))
+        acc, lo, hi = 0, lo_r, hi_r
+        for b in range(bins):
+            acc += int(hist[b])
+            at = F(lo_r + F(span * F(F(b) / F(bins))))
+            if acc >= lo_t and lo == lo_r:
+                lo = at
+            if acc >= hi_t:
+                hi = at
+                break
+        return lo, hi
+
+    def unique_values(self):
+        """The sorted distinct values of all tiles, halo cells included."""
+        return np.unique(np.concatenate([self.tile(r).ravel() for r in self.regions()]))
+
     def sum(self):
         """virtual_array_processing.cpp:225-251: float sums of interiors in
         row-major order (numpy's float32 cumsum adds sequentially), then over

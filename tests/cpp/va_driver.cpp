@@ -81,6 +81,11 @@ static int run_geom(const std::string &dir)
     dump(dir, "from_small_" + std::to_string(k), vb.to_array(seq));
     std::fprintf(txt, "red %d %a %a %a %a %a\n", k, va.sum(seq), va.min(dist), va.max(seq),
                  va.mean(dist), vb.sum(dist));
+    const glm::vec2 pr = va.range_percentile(0.05f, 0.95f, dist);
+    const glm::vec2 pr2 = va.range_percentile(0.2f, 0.7f, seq, 64);
+    const std::vector<float> uv = va.unique_values(dist);
+    std::fprintf(txt, "pct %d %a %a %a %a %zu %a %a\n", k, pr.x, pr.y, pr2.x, pr2.y, uv.size(),
+                 uv.front(), uv.back());
     // get / set and interpolation
     va.set(g.shape.x - 1, 0, 5.f);
     std::fprintf(txt, "get %d %a %a %a %a\n", k, va.get(g.shape.x / 2, g.shape.y / 3),

@@ -102,6 +102,21 @@ def test_smoothing_keeps_agreeing_overlaps():
     assert np.array_equal(va.to_array(), a)
 
 
+def test_percentiles_and_unique_values_of_a_known_distribution():
+    """The pattern takes the 101 values k/101 nearly equally often: its
+    p-percentile lies within a bin and a value of p, and its distinct values
+    are exactly those 101."""
+    va = V.VA((130, 97), (0, 1, 0, 1), (40, 33), 5)
+    a = pattern(130, 97)
+    va.from_array(a)
+    for p_lo, p_hi in ((0.05, 0.95), (0.25, 0.5)):
+        lo, hi = va.range_percentile(p_lo, p_hi)
+        assert abs(lo - p_lo * 100 / 101) <= 2 / 101 + 1 / 1024
+        assert abs(hi - p_hi * 100 / 101) <= 2 / 101 + 1 / 1024
+    assert np.array_equal(va.unique_values(), np.unique(a))
+    assert va.unique_values().size == 101
+
+
 def test_lerp_and_smoothstep5_closed_forms():
     assert V.smoothstep5(0.0) == 0 and V.smoothstep5(1.0) == 1 and V.smoothstep5(0.5) == 0.5
     assert V.lerp(2.0, 6.0, 0.0) == 2.0 and V.lerp(2.0, 6.0, 1.0) == 6.0
@@ -159,6 +174,11 @@ def test_cpp_virtual_array_matches_restatement(run, k):
     assert hexf(red[3]) == va.min() and hexf(red[4]) == va.max()
     assert hexf(red[5]) == va.mean()
     assert hexf(red[6]) == vb.sum()
+    pct = [ln.split() for ln in lines if ln.startswith("pct %d " % k)][0]
+    assert (hexf(pct[2]), hexf(pct[3])) == va.range_percentile(0.05, 0.95)
+    assert (hexf(pct[4]), hexf(pct[5])) == va.range_percentile(0.2, 0.7, 64)
+    uv = va.unique_values()
+    assert int(pct[6]) == uv.size and hexf(pct[7]) == uv[0] and hexf(pct[8]) == uv[-1]
     # get / set / interpolation
     va.set(nx - 1, 0, 5.0)
     got = [ln.split() for ln in lines if ln.startswith("get %d " % k)][0]
