@@ -306,18 +306,26 @@ int launch_thermal(hmg_ctx     *ctx,
                    hipStream_t  st,
                    float      **result)
 {
-  (void)ctx;
   float *a = z, *b = zscratch;
   int    done = 0;
   const int T = g_thermal_block_steps;
   if (!talus && !bedrock && T == 8)
   {
-    constexpr int TW = 64, TH = 16, TT = 8;
-    dim3          grd((nx + TW - 1) / TW, (ny + TH - 1) / TH);
+    // 64 x 16 tiles; grids too small to give every CU two of them (256^2
+    // gives 64) use 32 x 8 tiles, which recompute more halo per tile but
+    // occupy 4x as many CUs
+    constexpr int TT = 8;
+    const int     cus = ctx && ctx->num_cus > 0 ? ctx->num_cus : 256;
+    const bool    small = (long)((nx + 63) / 64) * ((ny + 15) / 16) < 2L * cus;
     while (iterations - done >= 2)
     {
       int s = iterations - done < TT ? iterations - done : TT;
-      k_thermal_blocked<TW, TH, TT><<<grd, 256, 0, st>>>(a, b, nx, ny, talus_scalar, s);
+      if (small)
+        k_thermal_blocked<32, 8, TT>
+            <<<dim3((nx + 31) / 32, (ny + 7) / 8), 256, 0, st>>>(a, b, nx, ny, talus_scalar, s);
+      else
+        k_thermal_blocked<64, 16, TT>
+            <<<dim3((nx + 63) / 64, (ny + 15) / 16), 256, 0, st>>>(a, b, nx, ny, talus_scalar, s);
       HMG_TRY(check_launch("thermal_blocked"));
       float *t = a;
       a = b;
