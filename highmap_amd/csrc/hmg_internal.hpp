@@ -168,6 +168,7 @@ struct hmg_session
   // process).  Results do not depend on the shape.  tune_pos < 0: done.
   int                     tune_pos = -1;
   std::vector<hipEvent_t> tune_ev;
+  std::vector<int>        tune_cand; // candidate shapes this session probes
   ~hmg_session()
   {
     for (hipEvent_t e : tune_ev) (void)hipEventDestroy(e);

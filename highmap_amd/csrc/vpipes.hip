@@ -1046,10 +1046,9 @@ std::mutex                                   g_shape_mu;
 std::map<TuneKey, std::pair<int, int>>       g_shape_cache;
 // (rows per wavefront, variant): 0 rows loaded when needed, 1 one row
 // ahead, 2 two rows ahead with the gathers one row ahead
-constexpr int TUNE_CAND[][2] = {{24, 1}, {32, 1}, {40, 1}, {47, 1}, {56, 1}, {64, 1},
-                                {47, 0}, {56, 0}, {47, 2}, {72, 2}, {94, 2}};
+constexpr int TUNE_CAND[][2] = {{8, 2},  {12, 1}, {16, 2}, {24, 1}, {32, 1}, {40, 1},
+                                {47, 1}, {56, 1}, {64, 1}, {47, 2}, {72, 2}, {94, 2}};
 constexpr int                                N_CAND = sizeof(TUNE_CAND) / sizeof(TUNE_CAND[0]);
-constexpr int                                N_PROBE = 1 + 2 * N_CAND; // warm-up + 2 rounds
 
 TuneKey tune_key(const hmg_session *s)
 {
@@ -1071,7 +1070,20 @@ static void tune_init(hmg_session *s)
       return;
     }
   }
-  s->tune_ev.assign(2 * N_PROBE, nullptr);
+  // the candidates that give every SIMD at least one wavefront (the
+  // shortest chunks if none does): longer chunks leave most of the GPU idle
+  // at this size and only slow the probes down
+  s->tune_cand.clear();
+  const int  strips = (s->nx + STRIP_OUT - 1) / STRIP_OUT;
+  const long simds = 4L * (s->ctx->num_cus > 0 ? s->ctx->num_cus : 256);
+  for (int c = 0; c < N_CAND; c++)
+    if ((long)strips * ((s->ny + TUNE_CAND[c][0] - 1) / TUNE_CAND[c][0]) >= simds)
+      s->tune_cand.push_back(c);
+  if (s->tune_cand.empty())
+    for (int c = 0; c < N_CAND; c++)
+      if (TUNE_CAND[c][0] <= 16) s->tune_cand.push_back(c);
+  const int n_probe = 1 + 2 * (int)s->tune_cand.size(); // warm-up + 2 rounds
+  s->tune_ev.assign(2 * n_probe, nullptr);
   for (hipEvent_t &e : s->tune_ev)
     if (hipEventCreate(&e) != hipSuccess)
     {
@@ -1094,15 +1106,17 @@ static void tune_stop(hmg_session *s)
 // Picks the best probed shape (blocks until the probes have run).
 static void tune_finish(hmg_session *s)
 {
-  float best = 0.f;
-  int   bi = -1;
-  if (hipEventSynchronize(s->tune_ev[2 * N_PROBE - 1]) == hipSuccess)
-    for (int c = 0; c < N_CAND; c++)
+  float     best = 0.f;
+  int       bi = -1;
+  const int nc = (int)s->tune_cand.size();
+  if (hipEventSynchronize(s->tune_ev.back()) == hipSuccess)
+    for (int k = 0; k < nc; k++)
     {
-      float t = -1.f;
+      const int c = s->tune_cand[k];
+      float     t = -1.f;
       for (int r = 0; r < 2; r++)
       {
-        const int q = 1 + r * N_CAND + c;
+        const int q = 1 + r * nc + k;
         float     ms = 0.f;
         if (hipEventElapsedTime(&ms, s->tune_ev[2 * q], s->tune_ev[2 * q + 1]) == hipSuccess)
           t = t < 0.f ? ms : std::min(t, ms);
@@ -1130,7 +1144,8 @@ static int dispatch_step(hmg_session *s);
 static int tuned_step(hmg_session *s)
 {
   if (s->tune_pos < 0) return dispatch_step(s);
-  if (s->tune_pos >= N_PROBE)
+  const int nc = (int)s->tune_cand.size();
+  if (s->tune_pos >= 1 + 2 * nc)
   {
     tune_finish(s);
     return dispatch_step(s);
@@ -1138,7 +1153,7 @@ static int tuned_step(hmg_session *s)
   const int q = s->tune_pos++;
   if (q > 0)
   {
-    const int c = (q - 1) % N_CAND;
+    const int c = s->tune_cand[(q - 1) % nc];
     s->chunk_rows = TUNE_CAND[c][0];
     s->prefetch = TUNE_CAND[c][1];
   }

@@ -426,10 +426,11 @@ int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d
  * needed, 1 = one row ahead, 2 = two rows ahead with each row's sediment
  * gathers issued one row ahead.  Results do not depend on either.  By
  * default a fused session of at least 2^20 cells tunes the shape online:
- * its first 23 hydraulic iterations each run one candidate shape between
- * two events, the next one keeps the fastest (a process-wide cache per
- * storage, shape, rain plane and exponent skips the probes for later
- * sessions).  Setting the shape ends the tuning. */
+ * its first hydraulic iterations (up to 25: a warm-up and two rounds over
+ * the candidate shapes that give every SIMD at least one wavefront) each
+ * run one candidate between two events, the next one keeps the fastest (a
+ * process-wide cache per storage, shape, rain plane and exponent skips the
+ * probes for later sessions).  Setting the shape ends the tuning. */
 int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch);
 /* The current shape; *tuning is 1 while the online probes still run. */
 int hmg_session_get_tuning(hmg_session *s, int *chunk_rows, int *prefetch, int *tuning);
