@@ -621,7 +621,9 @@ def main():
     ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
     ap.add_argument("--chunk-rows", type=int, default=0,
                     help="headline: fixed rows per wavefront (0: online tuning)")
-    ap.add_argument("--prefetch", type=int, default=1, choices=[0, 1])
+    ap.add_argument("--prefetch", type=int, default=1, choices=[0, 1, 2],
+                    help="with --chunk-rows: 0 one row ahead, 1 two rows ahead, "
+                         "2 two rows ahead with the sediment gathers one row ahead")
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--iters", type=int, default=500)
     ap.add_argument("--maps", type=int, default=0)
