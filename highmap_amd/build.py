@@ -88,9 +88,10 @@ def build(verbose=False):
               os.path.join(CSRC, "va_bench.cpp"), "-o", bench + ".tmp", "-L" + HERE,
               "-lhmap_gpu", "-lhmg", "-Wl,-rpath,$ORIGIN"])
         os.replace(bench + ".tmp", bench)
-        chk = os.path.join(ROOT, "tests", "cpp", "bf16_check")
-        _run([_hipcc()] + HIPCC_FLAGS + ["-x", "hip", chk + ".cpp", "-o", chk + ".tmp"])
-        os.replace(chk + ".tmp", chk)
+        for name in ("bf16_check", "arith_check"):
+            chk = os.path.join(ROOT, "tests", "cpp", name)
+            _run([_hipcc()] + HIPCC_FLAGS + ["-x", "hip", chk + ".cpp", "-o", chk + ".tmp"])
+            os.replace(chk + ".tmp", chk)
         for name in ("shim_driver", "io_driver", "va_driver"):
             drv = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
             if os.path.exists(drv):

@@ -396,6 +396,32 @@ __device__ __forceinline__ float hm_hypot(float x, float y)
   return sqrtf(x * x + y * y);
 }
 
+// sqrtf(x), correctly rounded, for x >= 2^-96 (and +0, +inf): the hardware
+// estimate, then the neighbours one ulp either side are chosen by the signs
+// of their fma residuals.  These are the steps the compiler emits for sqrtf,
+// less its scaling of inputs below 2^-96 and its select for special classes,
+// which leave such x unchanged.  For 0 <= x < 2^-96 the result is not
+// correctly rounded but stays below 2^-47.  Both facts are checked over
+// every float pattern on the GPU (tests/cpp/arith_check.cpp).
+__device__ __forceinline__ float sqrt_big(float x)
+{
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  float       r = (fmaf(-sm, s, x) <= 0.f) ? sm : s;
+  r = (fmaf(-sp, s, x) > 0.f) ? sp : r;
+  return r;
+}
+
+// sqrtf(x) for x >= +0 (a sum of squares): sqrt_big, unless a lane of the
+// wave holds a value in (0, 2^-96), then the full sequence for the whole
+// wave (a uniform branch).
+__device__ __forceinline__ float sqrt_sumsq(float x)
+{
+  if (__builtin_expect(__any(__float_as_uint(x) - 1u < 0x0f7fffffu), 0)) return sqrtf(x);
+  return sqrt_big(x);
+}
+
 // hydraulic_vpipes_erosion_pass (hydraulic_vpipes.cl:149-183) for one cell.
 // zr/zl/zt/zb and sr/sl/st/sb are the neighbour values right (i+1), left
 // (i-1), top (j+1), bottom (j-1).  KEXP1: k_discharge_exp is known to be 1,
@@ -420,10 +446,12 @@ __device__ __forceinline__ void vp_erosion(float        zc,
 {
   float dzx = 0.5f * (zr - zl + sr - sl);
   float dzy = 0.5f * (zt - zb + st - sb);
-  float talus = hm_hypot(dzx, dzy);
+  // The slope only enters through max(1e-3, nx * slope): below 2^-96 the
+  // exact root and sqrt_big's (both < 2^-47, nx < 2^31) give 1e-3 alike.
+  float talus = sqrt_big(dzx * dzx + dzy * dzy);
   float dzn = fmaxf(1e-3f, k.fnx * talus);
-  float salpha = fmaxf(0.01f, dzn / hm_hypot(1.f, dzn));
-  float speed = hm_hypot(u, v);
+  float salpha = fmaxf(0.01f, dzn / sqrt_big(1.f + dzn * dzn)); // argument >= 1
+  float speed = sqrt_sumsq(u * u + v * v);
   float depth = fminf(d2 / k.dmax, k.ratio);
   float discharge = depth * speed;
   float pw = (KEXP1 || k.kexp == 1.f) ? discharge : powf(discharge, k.kexp);
@@ -610,9 +638,24 @@ __device__ __forceinline__ Fix128 fix_term(float x)
 }
 
 // fix_term for x with bit pattern below 1.0f (finite, sign clear, x < 1):
-// then round-half-even(x * 2^64) < 2^64 and the high word is 0.  Equal to
-// fix_term(x).lo there (same shift, same rounding of the low bits).
+// then round-half-even(x * 2^64) < 2^64 and the high word is 0.  x * 2^64 is
+// exact (subnormal x included) and rint rounds it half-even to an integer
+// below 2^64, whose two 32-bit halves are exact in float.  Equal to
+// fix_term_unit_int for every such pattern (checked on the GPU by
+// tests/cpp/arith_check.cpp).
 __device__ __forceinline__ uint64_t fix_term_unit(float x)
+{
+  const float    t = __builtin_rintf(x * 0x1p64f);
+  const float    h = t * 0x1p-32f;
+  const float    hf = floorf(h);
+  const uint32_t hi = (uint32_t)hf;
+  const uint32_t lo = (uint32_t)((h - hf) * 0x1p32f);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// The same term by integer shifts, equal to fix_term(x).lo for x in [0, 1)
+// (same shift, same rounding of the low bits).
+__device__ __forceinline__ uint64_t fix_term_unit_int(float x)
 {
   const uint32_t b = __float_as_uint(x);
   const uint32_t e = b >> 23;
