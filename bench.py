@@ -84,13 +84,18 @@ def cpu_baseline(n, iters=20, reps=3):
     }
 
 
-def load_traffic(key):
+def load_traffic(key, chunk_rows, prefetch):
+    """HBM bytes per launch of the dominant kernel, from the rocprof
+    FETCH_SIZE/WRITE_SIZE runs in profiles/traffic.json for this workload and
+    launch shape (traffic depends on the shape: halo and pre-roll rows), or
+    None when that shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
         e = t[key]
-        return int(e["fetch_kib"] * e["fetch_scale"] * 1024 + e["write_kib"] * e["write_scale"] * 1024)
+        sh = e["shapes"]["%dx%d" % (chunk_rows, prefetch)]
+        return int(sh["fetch_kib"] * e["fetch_scale"] * 1024 + sh["write_kib"] * e["write_scale"] * 1024)
     except (OSError, KeyError, ValueError):
         return None
 
@@ -200,7 +205,8 @@ def run_headline(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic("vpipes_%s_%s_%d" % (args.mode, args.storage, n)),
+            "traffic": load_traffic("vpipes_%s_%s_%d" % (args.mode, args.storage, n),
+                                    chunk_rows, prefetch),
             "alg_bytes_per_launch": alg,
             "avg_launch_us": round(launch_s * 1e6, 2),
         },

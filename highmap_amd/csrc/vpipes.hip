@@ -1047,7 +1047,8 @@ std::map<TuneKey, std::pair<int, int>>       g_shape_cache;
 // (rows per wavefront, variant): 0 rows loaded when needed, 1 one row
 // ahead, 2 two rows ahead with the gathers one row ahead
 constexpr int TUNE_CAND[][2] = {{8, 2},  {12, 1}, {16, 2}, {24, 1}, {32, 1}, {40, 1},
-                                {47, 1}, {56, 1}, {64, 1}, {47, 2}, {72, 2}, {94, 2}};
+                                {47, 1}, {56, 1}, {64, 1}, {32, 2}, {40, 2}, {47, 2},
+                                {72, 2}, {94, 2}};
 constexpr int                                N_CAND = sizeof(TUNE_CAND) / sizeof(TUNE_CAND[0]);
 
 TuneKey tune_key(const hmg_session *s)
