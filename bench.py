@@ -622,7 +622,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--storage", default="f32", choices=sorted(STORAGE))
     ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
     ap.add_argument("--chunk-rows", type=int, default=0,

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "hmg.h"
@@ -168,7 +169,7 @@ struct hmg_session
   // process).  Results do not depend on the shape.  tune_pos < 0: done.
   int                     tune_pos = -1;
   std::vector<hipEvent_t> tune_ev;
-  std::vector<int>        tune_cand; // candidate shapes this session probes
+  std::vector<std::pair<int, int>> tune_cand; // (rows, variant) this session probes
   ~hmg_session()
   {
     for (hipEvent_t e : tune_ev) (void)hipEventDestroy(e);

@@ -1020,6 +1020,36 @@ static int dispatch_step(hmg_session *s)
   }
 }
 
+// Workgroups of the fused kernel variant (1: one-row load-ahead, 2: gathers
+// ahead) one CU holds at once for this session's storage and options, from
+// the kernel's register use (0 if the runtime cannot tell).
+template <class S>
+static int stream_blocks_per_cu_t(const hmg_session *s, int variant)
+{
+  const bool  rain = s->rain.ptr != nullptr, ke1 = s->k.kexp == 1.f;
+  const void *f = nullptr;
+  if (variant == 2)
+    f = rain ? (ke1 ? (const void *)k_vp_stream_ga<S, false, true> : (const void *)k_vp_stream_ga<S, false, false>)
+             : (ke1 ? (const void *)k_vp_stream_ga<S, true, true> : (const void *)k_vp_stream_ga<S, true, false>);
+  else
+    f = rain ? (ke1 ? (const void *)k_vp_stream<S, false, true, true> : (const void *)k_vp_stream<S, false, true, false>)
+             : (ke1 ? (const void *)k_vp_stream<S, true, true, true> : (const void *)k_vp_stream<S, true, true, false>);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64 * WPB, 0) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+static int stream_blocks_per_cu(const hmg_session *s, int variant)
+{
+  if (s->storage == HMG_STORAGE_BF16) return stream_blocks_per_cu_t<StBF16>(s, variant);
+  if (s->storage == HMG_STORAGE_F16) return stream_blocks_per_cu_t<StF16>(s, variant);
+  return stream_blocks_per_cu_t<StF32>(s, variant);
+}
+
 // ------------------------------------------------ online shape tuning
 //
 // The fused kernel's speed depends on rows per wavefront and load-ahead in
@@ -1082,13 +1112,33 @@ static void tune_init(hmg_session *s)
   // at this size and only slow the probes down
   s->tune_cand.clear();
   const int  strips = (s->nx + STRIP_OUT - 1) / STRIP_OUT;
-  const long simds = 4L * (s->ctx->num_cus > 0 ? s->ctx->num_cus : 256);
+  const int  cus = s->ctx->num_cus > 0 ? s->ctx->num_cus : 256;
+  const long simds = 4L * cus;
+  auto       items = [&](int rows) { return (long)strips * ((s->ny + rows - 1) / rows); };
   for (int c = 0; c < N_CAND; c++)
-    if ((long)strips * ((s->ny + TUNE_CAND[c][0] - 1) / TUNE_CAND[c][0]) >= simds)
-      s->tune_cand.push_back(c);
+    if (items(TUNE_CAND[c][0]) >= simds) s->tune_cand.push_back({TUNE_CAND[c][0], TUNE_CAND[c][1]});
   if (s->tune_cand.empty())
     for (int c = 0; c < N_CAND; c++)
-      if (TUNE_CAND[c][0] <= 16) s->tune_cand.push_back(c);
+      if (TUNE_CAND[c][0] <= 16) s->tune_cand.push_back({TUNE_CAND[c][0], TUNE_CAND[c][1]});
+  // Per variant, the shortest chunks whose work items all fit the
+  // wavefront slots the kernel's occupancy gives (one round of wavefronts,
+  // no nearly empty second round): at 4096^2 the gathers-ahead kernel holds
+  // 4 wavefronts per SIMD and 69 strips x 57 chunks of 72 rows fill 96% of
+  // them, while 68 rows (61 chunks) overflow into a second round and run
+  // 17% slower.
+  for (int v = 1; v <= 2; v++)
+  {
+    const int per_cu = stream_blocks_per_cu(s, v);
+    if (per_cu <= 0) continue;
+    const long slots = (long)per_cu * WPB * cus;
+    const long chunks = slots / strips;
+    if (chunks < 1) continue;
+    const int rows = (int)((s->ny + chunks - 1) / chunks);
+    if (rows < 8 || rows > 128) continue;
+    const std::pair<int, int> c{rows, v};
+    if (std::find(s->tune_cand.begin(), s->tune_cand.end(), c) == s->tune_cand.end())
+      s->tune_cand.push_back(c);
+  }
   const int n_probe = 1 + 2 * (int)s->tune_cand.size(); // warm-up + 2 rounds
   s->tune_ev.assign(2 * n_probe, nullptr);
   for (hipEvent_t &e : s->tune_ev)
@@ -1119,8 +1169,7 @@ static void tune_finish(hmg_session *s)
   if (hipEventSynchronize(s->tune_ev.back()) == hipSuccess)
     for (int k = 0; k < nc; k++)
     {
-      const int c = s->tune_cand[k];
-      float     t = -1.f;
+      float t = -1.f;
       for (int r = 0; r < 2; r++)
       {
         const int q = 1 + r * nc + k;
@@ -1131,14 +1180,14 @@ static void tune_finish(hmg_session *s)
       if (t >= 0.f && (bi < 0 || t < best))
       {
         best = t;
-        bi = c;
+        bi = k;
       }
     }
   (void)hipGetLastError();
   if (bi >= 0)
   {
-    s->chunk_rows = TUNE_CAND[bi][0];
-    s->prefetch = TUNE_CAND[bi][1];
+    s->chunk_rows = s->tune_cand[bi].first;
+    s->prefetch = s->tune_cand[bi].second;
     std::lock_guard<std::mutex> lk(g_shape_mu);
     g_shape_cache[tune_key(s)] = {s->chunk_rows, s->prefetch};
   }
@@ -1160,9 +1209,9 @@ static int tuned_step(hmg_session *s)
   const int q = s->tune_pos++;
   if (q > 0)
   {
-    const int c = s->tune_cand[(q - 1) % nc];
-    s->chunk_rows = TUNE_CAND[c][0];
-    s->prefetch = TUNE_CAND[c][1];
+    const std::pair<int, int> &c = s->tune_cand[(q - 1) % nc];
+    s->chunk_rows = c.first;
+    s->prefetch = c.second;
   }
   HMG_HIP(hipEventRecord(s->tune_ev[2 * q], s->ctx->stream));
   HMG_TRY(dispatch_step(s));
