@@ -1160,19 +1160,24 @@ static void tune_stop(hmg_session *s)
   s->tune_pos = -1;
 }
 
-// Picks the best probed shape (blocks until the probes have run).
+// Picks the best probed shape from the probes recorded so far (blocks until
+// they have run): all of them when the probes are complete, or at least one
+// round when a session ends while still probing (a drop-in call shorter
+// than the probes), so that the next session of the shape starts tuned.
 static void tune_finish(hmg_session *s)
 {
   float     best = 0.f;
   int       bi = -1;
   const int nc = (int)s->tune_cand.size();
-  if (hipEventSynchronize(s->tune_ev.back()) == hipSuccess)
+  const int done = s->tune_pos; // probes recorded: 0 .. done-1
+  if (done > 0 && hipEventSynchronize(s->tune_ev[2 * (done - 1) + 1]) == hipSuccess)
     for (int k = 0; k < nc; k++)
     {
       float t = -1.f;
       for (int r = 0; r < 2; r++)
       {
         const int q = 1 + r * nc + k;
+        if (q >= done) continue;
         float     ms = 0.f;
         if (hipEventElapsedTime(&ms, s->tune_ev[2 * q], s->tune_ev[2 * q + 1]) == hipSuccess)
           t = t < 0.f ? ms : std::min(t, ms);
@@ -1511,6 +1516,8 @@ int hmg_session_destroy(hmg_session *s)
   std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
+  // a session that ends after a full round of probes keeps their result
+  if (s->tune_pos > (int)s->tune_cand.size()) tune_finish(s);
   delete s;
   return HMG_OK;
 }

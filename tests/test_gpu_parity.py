@@ -275,6 +275,24 @@ def test_online_shape_tuning_keeps_results(hm, orc, storage):
         s.close()
 
 
+def test_tuning_kept_when_session_ends_mid_probe(hm, orc):
+    """A session that ends after one full round of probes (as a drop-in call
+    shorter than the probes does) caches the best shape probed so far, so
+    the next session of that shape starts tuned.  At 1024 x 1040 on 256 CUs
+    three shapes qualify (8, 12 and 16 rows); a warm-up probe and one round
+    are iterations 20-23."""
+    z0 = _terrain(orc, 1024, 1040, 13)
+    a = hm.Session(z0, storage=0, **VP_WORKLOAD)
+    a.vpipes(24)
+    assert a.tuning()[2], "still probing after 24 iterations"
+    a.close()
+    b = hm.Session(z0, storage=0, **VP_WORKLOAD)
+    ch, pf, probing = b.tuning()
+    assert not probing
+    assert (ch, pf) in {(8, 2), (12, 1), (16, 2)}
+    b.close()
+
+
 def test_plain_and_fused_agree_4096(hm, orc):
     """At full size: 6 iterations plain vs fused are identical, and the water
     volume stays at W0 (maintain_water_volume) to FP32 rounding."""
