@@ -2001,11 +2001,22 @@ int hmg_flow_simulation_dev(hmg_ctx     *ctx,
                                   (Flux4<StF32> *)F[0].ptr, n);
   HMG_TRY(check_launch("flow_init"));
 
-  // launch shape as for the sessions: about 16 wavefronts per CU, 8 to 64
-  // rows per wavefront
+  // launch shape: the chunk length whose work items just fit the wavefront
+  // slots the kernel's occupancy gives (one full round of wavefronts), 8 to
+  // 64 rows per wavefront; about 16 wavefronts per CU if the runtime cannot
+  // tell the occupancy
   const int  strips = (nx + STRIP_OUT - 1) / STRIP_OUT;
-  const long target = 16L * (ctx->num_cus > 0 ? ctx->num_cus : 256);
-  long       ch = ((long)ny * strips + target - 1) / target;
+  const int  cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+  int        per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_flow_stream<true>,
+                                                    64 * WPB, 0) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    per_cu = 0;
+  }
+  const long slots = per_cu > 0 ? (long)per_cu * WPB * cus : 16L * cus;
+  const long chunks = std::max(1L, slots / strips);
+  long       ch = (ny + chunks - 1) / chunks;
   ch = std::max(8L, std::min(64L, ch));
   const int n_items = strips * (int)((ny + ch - 1) / ch);
   const int G = (n_items + WPB - 1) / WPB;
