@@ -167,6 +167,7 @@ def run_headline(args):
     alg = ALG_BYTES[args.storage] * n * n
     achieved = alg / launch_s / 1e9
     chunk_rows, prefetch, _ = sess.tuning()
+    max_backtrace = sess.max_backtrace()
     sess.close()
 
     kernel = "k_vp_stream" if args.mode == "fused" else "k_vp_* (plain passes)"
@@ -195,6 +196,9 @@ def run_headline(args):
             "launch_shape": {"chunk_rows": chunk_rows, "prefetch": prefetch,
                              "source": "set" if args.chunk_rows else "online tuning"},
         },
+        # how far (cells) the last state's sediment backtraces reach: the
+        # gathers of this workload land hundreds of cells away
+        "max_backtrace_cells": round(max_backtrace, 2),
         "pct_hbm_peak_step": round(100.0 * (ALG_BYTES[args.storage] * n * n * iters +
                                             8 * n * n * args.steps) / elapsed / 1e9
                                    / HBM_PEAK_GBS, 2),
@@ -400,11 +404,21 @@ def run_thermal256(args):
     c1 = cpu_bench("thermal", 256, 256, 50, 5, 1)
     cn = cpu_bench("thermal", 256, 256, 50, 5)
     cells = 256 * 256 * 50
+
+    def interior_sum(a):
+        return float(a[1:-1, 1:-1].astype(np.float64).sum())
+
+    s0 = interior_sum(z0)
     return {"metric": "Gcell-iter/s thermal 256^2 x 50 (drop-in)",
             "value": round(cells / call_s / 1e9, 4), "unit": "Gcell-iter/s", "n_gpus": 1,
             "device_resident_gcell_iter_s": round(cells / dev_s / 1e9, 4),
             "bitexact_vs_fp32_oracle": bool(np.array_equal(z, O.thermal(z0, 2.0 / 256, 50))),
             "max_abs_err_rel_range_vs_fp64": float(np.abs(z - o64).max() / (o64.max() - o64.min())),
+            # relative change of the interior sum over the 50 steps (the
+            # exchange with border cells moves height across the interior's
+            # edge), for the GPU result and the FP64 oracle
+            "interior_sum_drift": (interior_sum(z) - s0) / s0,
+            "interior_sum_drift_fp64": (interior_sum(o64) - s0) / s0,
             "cpu_1_thread_gcell_iter_s": c1["gcell_iter_s"],
             "cpu_all_threads_gcell_iter_s": cn["gcell_iter_s"], "cpu_threads": cn["threads"]}
 

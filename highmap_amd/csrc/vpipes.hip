@@ -1422,6 +1422,68 @@ int hmg_session_export(hmg_session *s, float *z_host, float *d_host, float *s_ho
   return HMG_OK;
 }
 
+// max(|u|, |v|) over n cells as the bit pattern of a non-negative float: for
+// those, unsigned order is float order (NaN above +inf), so the result does
+// not depend on the order of the atomic maxima.
+__global__ void __launch_bounds__(256) k_absmax2(const float *__restrict__ u,
+                                                 const float *__restrict__ v,
+                                                 size_t n,
+                                                 unsigned int *out)
+{
+  unsigned int m = 0;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+  {
+    m = max(m, __float_as_uint(u[c]) & 0x7fffffffu);
+    m = max(m, __float_as_uint(v[c]) & 0x7fffffffu);
+  }
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ void k_zero_u32(unsigned int *p)
+{
+  *p = 0u;
+}
+
+int hmg_session_max_backtrace(hmg_session *s, double *max_backtrace)
+{
+  clear_error();
+  if (!s || !max_backtrace) return set_error(HMG_ERR_INVALID, "null argument");
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  HMG_TRY(ctx_enter(s->ctx));
+  const size_t n = (size_t)s->nx * s->ny;
+  hipStream_t  st = s->ctx->stream;
+  DevBuf       out, mx;
+  HMG_TRY(out.get(s->ctx->pool, n * 16));
+  HMG_TRY(mx.get(s->ctx->pool, sizeof(unsigned int)));
+  float *o = out.as<float>();
+  int    rc;
+  switch (s->storage)
+  {
+  case HMG_STORAGE_BF16: rc = session_export<StBF16>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+  case HMG_STORAGE_F16: rc = session_export<StF16>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+  default: rc = session_export<StF32>(s, o, o + n, o + 2 * n, o + 3 * n); break;
+  }
+  HMG_TRY(rc);
+  unsigned int *m = mx.as<unsigned int>();
+  k_zero_u32<<<1, 1, 0, st>>>(m);
+  HMG_TRY(check_launch("zero_u32"));
+  int nb = (int)((n + 255) / 256);
+  if (nb > 2048) nb = 2048;
+  k_absmax2<<<nb, 256, 0, st>>>(o + 2 * n, o + 3 * n, n, m);
+  HMG_TRY(check_launch("absmax2"));
+  unsigned int h = 0;
+  HMG_HIP(hipMemcpyAsync(&h, m, sizeof(h), hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  float mf;
+  std::memcpy(&mf, &h, sizeof(mf));
+  // dt * max|u| in FP64: the product of two floats is exact there, and the
+  // maximum of dt |u| over the cells is dt times the maximum of |u| (dt > 0)
+  *max_backtrace = (double)s->k.dt * (double)mf;
+  return HMG_OK;
+}
+
 int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d)
 {
   clear_error();

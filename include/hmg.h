@@ -421,16 +421,24 @@ int hmg_session_export(hmg_session *s,
                        float       *v_host);
 /* Waits, then returns sum(z), sum(s), sum(d) in FP64 (mass check). */
 int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d);
+/* Waits, then returns max over the cells of dt * max(|u|, |v|) for the
+ * state's velocities (those hmg_session_export returns), in FP64: how far, in
+ * cells, the next iteration's sediment backtrace (the semi-Lagrangian
+ * transport, hydraulic_vpipes.cl:204-210) reaches; the reference computes no
+ * such bound (its dt is fixed).  NaN if a velocity is NaN. */
+int hmg_session_max_backtrace(hmg_session *s, double *max_backtrace);
 /* Launch shape of the fused kernel: rows each wavefront streams through
  * and the variant, `prefetch`: 0 = each row's loads issued when the row is
  * needed, 1 = one row ahead, 2 = two rows ahead with each row's sediment
  * gathers issued one row ahead.  Results do not depend on either.  By
  * default a fused session of at least 2^20 cells tunes the shape online:
- * its first hydraulic iterations (up to 25: a warm-up and two rounds over
- * the candidate shapes that give every SIMD at least one wavefront) each
- * run one candidate between two events, the next one keeps the fastest (a
- * process-wide cache per storage, shape, rain plane and exponent skips the
- * probes for later sessions).  Setting the shape ends the tuning. */
+ * from its 20th hydraulic iteration on, a warm-up and two rounds over the
+ * candidate shapes (those that give every SIMD at least one wavefront, and
+ * per variant the chunk length that fills the kernel's wavefront slots)
+ * each run one candidate between two events, the next one keeps the
+ * fastest (a process-wide cache per storage, shape, rain plane and exponent
+ * skips the probes for later sessions; a session that ends after a full
+ * round caches its best so far).  Setting the shape ends the tuning. */
 int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch);
 /* The current shape; *tuning is 1 while the online probes still run. */
 int hmg_session_get_tuning(hmg_session *s, int *chunk_rows, int *prefetch, int *tuning);

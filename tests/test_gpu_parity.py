@@ -293,6 +293,31 @@ def test_tuning_kept_when_session_ends_mid_probe(hm, orc):
     b.close()
 
 
+@pytest.mark.parametrize("storage", [0, 1])
+def test_session_sums_and_max_backtrace(hm, orc, storage):
+    """The device reductions over a session's state: the sums of z, s and d
+    (exact fixed point, returned in FP64) equal the exactly rounded FP64
+    sums of the exported planes to 1e-12 relative; the max backtrace is
+    dt * max(|u|, |v|) of the exported velocities, exactly (an FP64 product
+    of two floats); before any iteration it is 0."""
+    import math
+
+    z0 = _terrain(orc, 200, 136, 21)
+    s = hm.Session(z0, storage=storage, **VP_WORKLOAD)
+    assert s.max_backtrace() == 0.0
+    s.vpipes(40)
+    z, d, sd, u, v = s.export()
+    gz, gs, gd = s.sums()
+    for g, a in ((gz, z), (gs, sd), (gd, d)):
+        ref = math.fsum(a.astype(np.float64).ravel())
+        assert abs(g - ref) <= 1e-12 * max(1.0, abs(ref)), (g, ref)
+    dt = np.float64(np.float32(VP_WORKLOAD["dt"]))
+    want = dt * max(np.abs(u).max(), np.abs(v).max()).astype(np.float64)
+    assert s.max_backtrace() == want
+    assert want > 0.0
+    s.close()
+
+
 def test_plain_and_fused_agree_4096(hm, orc):
     """At full size: 6 iterations plain vs fused are identical, and the water
     volume stays at W0 (maintain_water_volume) to FP32 rounding."""
