@@ -540,20 +540,6 @@ class Batch:
         self.shapes.append((ny, nx))
         return mid.value
 
-    def sums(self):
-        """(sum z, sum s, sum d) of the state hydraulic_vpipes would return,
-        summed exactly on the device (fixed point) and given in FP64."""
-        a, b, c = C.c_double(), C.c_double(), C.c_double()
-        _check(lib().hmg_session_sums(self.handle, C.byref(a), C.byref(b), C.byref(c)))
-        return a.value, b.value, c.value
-
-    def max_backtrace(self):
-        """max over the cells of dt * max(|u|, |v|) (FP64): how many cells the
-        next sediment backtrace reaches."""
-        a = C.c_double()
-        _check(lib().hmg_session_max_backtrace(self.handle, C.byref(a)))
-        return a.value
-
     def set_tuning(self, chunk_rows, prefetch=1):
         _check(lib().hmg_batch_set_tuning(self.handle, int(chunk_rows), int(prefetch)))
 

@@ -64,6 +64,9 @@ struct hmg_server
   long long                     next_id = 1;
   double                        cell_iters = 0, busy_s = 0;
   long long                     ticks = 0;
+  // launch shape of the batched hydraulic launches, tuned online (tick())
+  int                           probe = 0, rows = 48, variant = 1;
+  double                        cost[6] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace
@@ -97,6 +100,15 @@ int admit(hmg_server *sv, Job *j)
   return hmg_session_create_dev(ctx, t.as<float>(), d.nx, d.ny, &p, nullptr, sv->storage, &j->s);
 }
 
+// Candidate shapes of the batched hydraulic launches (rows per wavefront,
+// variant: 1 one row ahead, 2 gathers ahead).  The server's first 2 x 6
+// ticks with hydraulic work run them in turn, each timed on the device; the
+// cost of a tick is its device time per cell-iteration (so ticks with
+// different job mixes compare), the cheaper of a shape's two ticks counts,
+// and the cheapest shape is kept.  Results do not depend on the shape.
+constexpr int SERVE_CAND[6][2] = {{32, 1}, {48, 1}, {64, 1}, {40, 2}, {56, 2}, {72, 2}};
+constexpr int N_SERVE_CAND = 6;
+
 // One tick over the active jobs; returns the cell-iterations done.
 double tick(hmg_server *sv, std::vector<Job *> &act)
 {
@@ -119,8 +131,28 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
   int rc = HMG_OK;
   if (!vs.empty())
   {
-    rc = batch_advance(ctx, vs.data(), vi.data(), (int)vs.size(), 0, vs[0]->chunk_rows, 1,
-                       nullptr);
+    const bool probing = sv->probe < 2 * N_SERVE_CAND;
+    const int  ci = sv->probe % N_SERVE_CAND;
+    const int  rows = probing ? SERVE_CAND[ci][0] : sv->rows;
+    const int  variant = probing ? SERVE_CAND[ci][1] : sv->variant;
+    float      ms = 0.f;
+    rc = batch_advance(ctx, vs.data(), vi.data(), (int)vs.size(), 0, rows, variant,
+                       probing ? &ms : nullptr);
+    if (probing && rc == HMG_OK)
+    {
+      double cells = 0;
+      for (size_t q = 0; q < vj.size(); q++) cells += (double)vj[q]->desc.nx * vj[q]->desc.ny * vi[q];
+      const double c = cells > 0 ? ms / cells : 0.0;
+      if (sv->probe < N_SERVE_CAND || c < sv->cost[ci]) sv->cost[ci] = c;
+      if (++sv->probe == 2 * N_SERVE_CAND)
+      {
+        int b = 0;
+        for (int q = 1; q < N_SERVE_CAND; q++)
+          if (sv->cost[q] < sv->cost[b]) b = q;
+        sv->rows = SERVE_CAND[b][0];
+        sv->variant = SERVE_CAND[b][1];
+      }
+    }
     for (size_t q = 0; q < vj.size(); q++)
     {
       Job *j = vj[q];

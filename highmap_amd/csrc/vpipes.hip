@@ -687,7 +687,7 @@ struct VpMapDesc
   int          t_start, it0, iters;
 };
 
-template <class S, bool URAIN, bool PF, bool KEXP1>
+template <class S, bool URAIN, bool PF, bool KEXP1, bool GA = false>
 __global__ void __launch_bounds__(64 * WPB) k_vp_batch(const VpMapDesc *__restrict__ maps,
                                                   int n_maps,
                                                   int n_items,
@@ -719,7 +719,7 @@ __global__ void __launch_bounds__(64 * WPB) k_vp_batch(const VpMapDesc *__restri
   Fix128 *prev = m.slots + ((mi + 2) % 3) * NSLOT;
   Fix128 *cur = m.slots + (mi % 3) * NSLOT;
   Fix128 *nxt = m.slots + ((mi + 1) % 3) * NSLOT;
-  vp_stream_item<S, URAIN, PF, KEXP1>((const float *)m.z[a], (const T *)m.D[a],
+  vp_stream_item<S, URAIN, PF, KEXP1, false, GA>((const float *)m.z[a], (const T *)m.D[a],
                                       (const Flux4<S> *)m.F[a], (const T *)m.Sp[a], m.rain,
                                       (float *)m.z[b], (T *)m.D[b], (Flux4<S> *)m.F[b],
                                       (T *)m.Sp[b], prev, cur, nxt, item - m.item_base, m.strips,
@@ -1621,7 +1621,7 @@ namespace hmg
 // storage type.  With use_graph the whole launch sequence is captured in a
 // hipGraph and submitted as one graph.  Returns the elapsed device time of
 // the launches in *ms (if not null).
-template <class S, bool PF, bool KEXP1>
+template <class S, bool PF, bool KEXP1, bool GA = false>
 static int batch_advance_t(hmg_ctx           *ctx,
                            hmg_session *const *ss,
                            const int          *iters,
@@ -1692,9 +1692,9 @@ static int batch_advance_t(hmg_ctx           *ctx,
     if (G == 0) continue;
     const VpMapDesc *tab = table.as<VpMapDesc>();
     if (any_rain)
-      k_vp_batch<S, false, PF, KEXP1><<<G, 64 * WPB, 0, st>>>(tab, active, items, t);
+      k_vp_batch<S, false, PF, KEXP1, GA><<<G, 64 * WPB, 0, st>>>(tab, active, items, t);
     else
-      k_vp_batch<S, true, PF, KEXP1><<<G, 64 * WPB, 0, st>>>(tab, active, items, t);
+      k_vp_batch<S, true, PF, KEXP1, GA><<<G, 64 * WPB, 0, st>>>(tab, active, items, t);
   }
   int rc = check_launch("vp_batch");
   if (use_graph)
@@ -1758,11 +1758,16 @@ int batch_advance(hmg_ctx           *ctx,
       return set_error(HMG_ERR_INVALID, "iteration count too large");
   }
   if (chunk_rows < 1) return set_error(HMG_ERR_INVALID, "chunk_rows must be >= 1");
+  if (prefetch < 0 || prefetch > 2) return set_error(HMG_ERR_INVALID, "prefetch must be 0, 1 or 2");
 #define HMG_BATCH(ST)                                                          \
-  (prefetch ? (ke1 ? batch_advance_t<ST, true, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms) \
-                   : batch_advance_t<ST, true, false>(ctx, ss, iters, n, use_graph, chunk_rows, ms)) \
-            : (ke1 ? batch_advance_t<ST, false, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms) \
-                   : batch_advance_t<ST, false, false>(ctx, ss, iters, n, use_graph, chunk_rows, ms)))
+  (prefetch == 2                                                               \
+       ? (ke1 ? batch_advance_t<ST, true, true, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms) \
+              : batch_advance_t<ST, true, false, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms)) \
+   : prefetch                                                                  \
+       ? (ke1 ? batch_advance_t<ST, true, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms) \
+              : batch_advance_t<ST, true, false>(ctx, ss, iters, n, use_graph, chunk_rows, ms)) \
+       : (ke1 ? batch_advance_t<ST, false, true>(ctx, ss, iters, n, use_graph, chunk_rows, ms) \
+              : batch_advance_t<ST, false, false>(ctx, ss, iters, n, use_graph, chunk_rows, ms)))
   switch (storage)
   {
   case HMG_STORAGE_BF16: return HMG_BATCH(StBF16);
@@ -1844,7 +1849,7 @@ int hmg_batch_set_tuning(hmg_batch *b, int chunk_rows, int prefetch)
 {
   clear_error();
   if (!b) return set_error(HMG_ERR_INVALID, "null batch");
-  if (chunk_rows < 1 || (prefetch != 0 && prefetch != 1))
+  if (chunk_rows < 1 || prefetch < 0 || prefetch > 2)
     return set_error(HMG_ERR_INVALID, "bad tuning values");
   b->chunk_rows = chunk_rows;
   b->prefetch = prefetch;
@@ -1883,7 +1888,7 @@ int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefet
   float            best = 1e30f;
   int              best_rows = b->chunk_rows, best_pf = b->prefetch;
   std::vector<int> step(b->maps.size());
-  for (int pf = 0; pf < 2; pf++)
+  for (int pf = 0; pf < 3; pf++) // rows loaded when needed, one ahead, gathers ahead
     for (int rows : cand_rows)
     {
       int min_left = 1 << 30;

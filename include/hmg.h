@@ -468,7 +468,8 @@ int hmg_batch_add(hmg_batch   *b,
                   const float *rain_host,
                   int          iterations,
                   int         *map_id);
-/* Launch shape of the batched kernel (rows per wavefront, load-ahead). */
+/* Launch shape of the batched kernel: rows per wavefront and variant as in
+ * hmg_session_set_tuning (0, 1, or 2 = gathers ahead). */
 int hmg_batch_set_tuning(hmg_batch *b, int chunk_rows, int prefetch);
 /* Picks the launch shape by timing `probe` iterations of each candidate on
  * the live maps (they count toward each map's iterations; results do not
@@ -488,7 +489,9 @@ int hmg_batch_destroy(hmg_batch *b);
  * arrival order while their state fits in an HBM budget, keep their state
  * resident on the device, and advance together (batched vpipes launches,
  * then due thermal steps) tick by tick; each finished job's z is copied to
- * the caller's buffer.  A job's result equals the same job run alone.
+ * the caller's buffer.  A job's result equals the same job run alone.  The
+ * batched launches' shape is tuned online on the first ticks (device time
+ * per cell-iteration of six candidate shapes, two ticks each).
  * No reference counterpart (HighMap runs one call at a time). */
 typedef struct hmg_server hmg_server;
 
