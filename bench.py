@@ -411,6 +411,22 @@ def run_thermal256(args):
                                            50, None))
     dev_s = (time.perf_counter() - t0) / reps
     o64 = O.thermal(z0.astype(np.float64), 2.0 / 256, 50, np.float64)
+    # the same call at 4096^2, device-resident (LDS-blocked kernel, 8 steps
+    # per launch): events around the call, best of 3
+    big = 4096
+    zb = device_terrain(torch, hm, ctx, big, big, 1)
+    st = torch.cuda.ExternalStream(ctx.stream)
+    big_ms = None
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        hm._check(hm.lib().hmg_thermal_dev(ctx.handle, zb.data_ptr(), big, big, None, 2.0 / big,
+                                           50, None))
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        big_ms = ms if big_ms is None else min(big_ms, ms)
+    big_rate = big * big * 50 / (big_ms / 1e3) / 1e9
     c1 = cpu_bench("thermal", 256, 256, 50, 5, 1)
     cn = cpu_bench("thermal", 256, 256, 50, 5)
     cells = 256 * 256 * 50
@@ -429,6 +445,11 @@ def run_thermal256(args):
             # edge), for the GPU result and the FP64 oracle
             "interior_sum_drift": (interior_sum(z) - s0) / s0,
             "interior_sum_drift_fp64": (interior_sum(o64) - s0) / s0,
+            # 4096^2 x 50 resident: 8 B per cell-step is the one-step-per-pass
+            # HBM minimum; the blocked kernel moves less (8 steps per pass)
+            "resident_4096_gcell_iter_s": round(big_rate, 2),
+            "resident_4096_ms": round(big_ms, 3),
+            "resident_4096_gbs_at_8B_per_cell_step": round(big_rate * 8, 1),
             "cpu_1_thread_gcell_iter_s": c1["gcell_iter_s"],
             "cpu_all_threads_gcell_iter_s": cn["gcell_iter_s"], "cpu_threads": cn["threads"]}
 

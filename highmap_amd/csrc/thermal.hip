@@ -309,7 +309,14 @@ int launch_thermal(hmg_ctx     *ctx,
   float *a = z, *b = zscratch;
   int    done = 0;
   const int T = g_thermal_block_steps;
-  if (!talus && !bedrock && T == 8)
+  // The blocked kernel saves launches and HBM passes but recomputes an
+  // 8-cell halo per tile; both kernels are VALU-bound, so it pays only where
+  // launches dominate.  50 steps, device-resident (ms): 256^2 blocked 3x
+  // faster; 512^2 0.196 blocked / 0.183 one step per launch; 1024^2 0.409 /
+  // 0.347; 4096^2 3.56 / 2.45.  Grids of 2^18 cells and more step one
+  // Jacobi step per launch.
+  const bool blocked_pays = (long)nx * ny < (1L << 18);
+  if (!talus && !bedrock && T == 8 && blocked_pays)
   {
     // 64 x 16 tiles; grids too small to give every CU two of them (256^2
     // gives 64) use 32 x 8 tiles, which recompute more halo per tile but

@@ -69,7 +69,7 @@ def test_thermal_256_talus_2_over_256_50_iterations(hm, orc):
 
 @pytest.mark.parametrize("nx,ny,iters", [(3, 3, 4), (5, 7, 3), (130, 67, 0), (130, 67, 1),
                                          (130, 67, 8), (131, 70, 9), (64, 16, 17),
-                                         (300, 200, 25)])
+                                         (300, 200, 25), (520, 512, 10)])
 def test_thermal_shapes_and_step_counts(hm, orc, nx, ny, iters):
     z0 = _terrain(orc, nx, ny, 5) if nx > 4 else np.random.default_rng(1).random(
         (ny, nx), dtype=np.float32)
