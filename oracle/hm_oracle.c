@@ -203,6 +203,14 @@ static float orc_corner(float cx, float cy, float fseed, float dx, float dy)
   return gx * dx + gy * dy;
 }
 
+/* grad22f alone (for the known-answer tests). */
+void orc_grad22f(float px, float py, float fseed, float *gx, float *gy)
+{
+  float angle = 6.2831853f * orc_hash12f(px, py, fseed);
+  *gx = orc_cosf(angle);
+  *gy = orc_sinf(angle);
+}
+
 static float orc_fade(float x)
 {
   /* smoothstep5 (_common_math.cl:126-129): x*x*x*(x*(6x-15)+10) */

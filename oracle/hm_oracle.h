@@ -76,6 +76,7 @@ float    orc_sinf(float x);
 float    orc_cosf(float x);
 float    orc_fractf(float x);
 float    orc_hash12f(float px, float py, float fseed);
+void     orc_grad22f(float px, float py, float fseed, float *gx, float *gy);
 float    orc_base_perlin(float px, float py, float fseed, int per_x, int per_y);
 /* Single-octave value of hmap::NoiseType `type` (0 for types the kernels do
  * not implement), and whether the kernels implement it. */
@@ -263,6 +264,9 @@ void orc_vp_erosion_f32(const float *z, const float *s, const float *d2,
                         int nx, int ny, const orc_vp_params *p);
 void orc_vp_transport_f32(const float *u, const float *v, const float *s,
                           float *so, int nx, int ny, float dt);
+void orc_vp_erosion_f64(const double *z, const double *s, const double *d2,
+                        const double *u, const double *v, double *zo, double *so,
+                        int nx, int ny, const orc_vp_params *p);
 
 #ifdef __cplusplus
 }

@@ -132,6 +132,8 @@ def lib():
     L.orc_vp_water_f32.argtypes = [F32P] * 8 + [C.c_int, C.c_int, pp]
     L.orc_vp_erosion_f32.argtypes = [F32P] * 7 + [C.c_int, C.c_int, pp]
     L.orc_vp_transport_f32.argtypes = [F32P] * 4 + [C.c_int, C.c_int, C.c_float]
+    L.orc_vp_erosion_f64.argtypes = [F64P] * 7 + [C.c_int, C.c_int, pp]
+    L.orc_grad22f.argtypes = [C.c_float] * 3 + [C.POINTER(C.c_float)] * 2
     L.orc_thermal_auto_bedrock_steps_f32.argtypes = [F32P, C.c_int, C.c_int, C.c_void_p,
                                                      C.c_float, C.c_int]
     L.orc_thermal_auto_bedrock_steps_f64.argtypes = [F64P, C.c_int, C.c_int, C.c_void_p,
@@ -202,6 +204,28 @@ def noise(nx, ny, kw=(4.0, 4.0), seed=0, noise_type=1, noise_x=None, noise_y=Non
     lib().orc_noise_f32(out, nx, ny, noise_type, kw[0], kw[1], seed & 0xFFFFFFFF,
                         _ptr(keep[0]), _ptr(keep[1]), _ptr(bb), _ptr(pe))
     return out
+
+
+def hash12f(px, py, fseed):
+    """hash12f at one point (float32 inputs)."""
+    return float(lib().orc_hash12f(px, py, fseed))
+
+
+def grad22f(px, py, fseed):
+    """grad22f at one point: (cos, sin) of 6.2831853 * hash12f."""
+    gx, gy = C.c_float(), C.c_float()
+    lib().orc_grad22f(px, py, fseed, C.byref(gx), C.byref(gy))
+    return gx.value, gy.value
+
+
+def vp_erosion(z, s, d2, u, v, params, dtype=np.float32):
+    """One erosion pass of hydraulic_vpipes on given planes; returns (z, s)."""
+    planes = [np.ascontiguousarray(a, dtype) for a in (z, s, d2, u, v)]
+    ny, nx = planes[0].shape
+    zo, so = np.empty_like(planes[0]), np.empty_like(planes[0])
+    fn = lib().orc_vp_erosion_f32 if dtype == np.float32 else lib().orc_vp_erosion_f64
+    fn(*planes, zo, so, nx, ny, C.byref(params))
+    return zo, so
 
 
 def base_noise(noise_type, px, py, fseed, period=(0, 0)):
