@@ -91,10 +91,10 @@ def test_device_code_has_no_single_rounding_mixed_precision_ops():
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not os.path.exists(hipcc):
         pytest.skip("hipcc not available")
-    for src in ("noise.hip", "thermal.hip", "vpipes.hip", "batch.hip"):
+    assert hb.HIP_SOURCES, "no HIP sources listed"
+    for src in hb.HIP_SOURCES:
         path = os.path.join(hb.CSRC, src)
-        if not os.path.exists(path):
-            continue
+        assert os.path.exists(path), src
         flags = [f for f in hb.HIPCC_FLAGS if f != "-fPIC"]
         asm = subprocess.run([hipcc] + flags + ["-x", "hip", "-S", "--cuda-device-only", path,
                                                 "-o", "-"], capture_output=True, text=True,

@@ -68,7 +68,7 @@ def test_autotune_keeps_results(hm, orc):
 
 
 @pytest.mark.parametrize("rows", [8, 33])
-@pytest.mark.parametrize("storage", [0, 1])
+@pytest.mark.parametrize("storage", [0, 1, 2])
 def test_batch_gathers_ahead_equals_single_sessions(hm, orc, rows, storage):
     """The gathers-ahead variant of the batched kernel (rows loaded two
     ahead, sediment gathers one row early) gives the same maps."""
