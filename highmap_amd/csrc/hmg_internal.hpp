@@ -139,6 +139,10 @@ int launch_thermal(hmg_ctx     *ctx,
                    float      **result = nullptr);
 int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
 
+// HMG_ERR_NONFINITE if the n floats at p (device) hold a NaN or an
+// infinity; waits for the context stream (vpipes.hip).
+int check_finite_f32(hmg_ctx *ctx, const float *p, size_t n);
+
 // Per-block min and max of a[0..n) into mm[2 + 2b], mm[3 + 2b] for the nblk
 // blocks of the launch (nblk <= 1024; mm holds 2 + 2 * nblk floats).
 int launch_minmax(const float *a, size_t n, float *mm, int nblk, hipStream_t st);
@@ -181,6 +185,10 @@ namespace hmg
 
 // Advances sessions[i] by iters[i] hydraulic iterations with batched
 // launches (vpipes.hip).  *ms (may be null) receives the device time.
+// HMG_ERR_NONFINITE if the session's state holds a NaN or an infinity;
+// waits for the context stream (vpipes.hip).
+int session_check_finite(hmg_session *s);
+
 int batch_advance(hmg_ctx           *ctx,
                   hmg_session *const *ss,
                   const int          *iters,

@@ -15,6 +15,7 @@
 // running.  No reference counterpart: HighMap runs one call at a time.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -64,9 +65,14 @@ struct hmg_server
   long long                     next_id = 1;
   double                        cell_iters = 0, busy_s = 0;
   long long                     ticks = 0;
-  // launch shape of the batched hydraulic launches, tuned online (tick())
-  int                           probe = 0, rows = 48, variant = 1;
-  double                        cost[6] = {0, 0, 0, 0, 0, 0};
+  // launch shape of the batched hydraulic launches, tuned online per size
+  // class of a tick's hydraulic work (tick())
+  struct Shape
+  {
+    int    probe = 0, rows = 48, variant = 1;
+    double cost[6] = {0, 0, 0, 0, 0, 0};
+  };
+  std::map<int, Shape>          shapes;
 };
 
 namespace
@@ -100,14 +106,47 @@ int admit(hmg_server *sv, Job *j)
   return hmg_session_create_dev(ctx, t.as<float>(), d.nx, d.ny, &p, nullptr, sv->storage, &j->s);
 }
 
+// Publishes a job's state to pollers: state, t_admit and t_done are read
+// under sv->mu by other threads, and the lock orders the result copy into
+// the caller's buffer before a poller can see HMG_JOB_DONE.
+void set_state(hmg_server *sv, Job *j, int state)
+{
+  std::lock_guard<std::mutex> lk(sv->mu);
+  j->state = state;
+}
+
 // Candidate shapes of the batched hydraulic launches (rows per wavefront,
-// variant: 1 one row ahead, 2 gathers ahead).  The server's first 2 x 6
-// ticks with hydraulic work run them in turn, each timed on the device; the
-// cost of a tick is its device time per cell-iteration (so ticks with
-// different job mixes compare), the cheaper of a shape's two ticks counts,
-// and the cheapest shape is kept.  Results do not depend on the shape.
+// variant: 1 one row ahead, 2 gathers ahead).  Ticks are grouped in size
+// classes, floor(log2(cells of the tick's hydraulic jobs)); the first 2 x 6
+// ticks of a class run the candidates in turn, each timed on the device; the
+// cost of a tick is its device time per cell-iteration, the cheaper of a
+// shape's two ticks counts, and the cheapest shape is kept for the class.
+// Only ticks whose jobs are all past their first TUNE_AFTER_IT hydraulic
+// iterations probe: while the water spreads from rest the sediment
+// backtraces are short and hit the cache, which ranks the shapes
+// differently from the rest of a run.  Results do not depend on the shape.
 constexpr int SERVE_CAND[6][2] = {{32, 1}, {48, 1}, {64, 1}, {40, 2}, {56, 2}, {72, 2}};
 constexpr int N_SERVE_CAND = 6;
+constexpr int TUNE_AFTER_IT = 20;
+
+// Rows per wavefront for a tick: at most `rows`, shortened (not below 8)
+// while the tick's maps would give fewer than one wavefront per SIMD, so
+// that a few small maps still spread over the device.
+int capped_rows(const hmg_server *sv, const std::vector<Job *> &vj, int rows)
+{
+  const long target = 4L * (sv->ctx->num_cus > 0 ? sv->ctx->num_cus : 256);
+  for (;;)
+  {
+    long items = 0;
+    for (const Job *j : vj)
+    {
+      const long strips = (j->desc.nx + 59) / 60; // 60 columns per wavefront
+      items += strips * ((j->desc.ny + rows - 1) / rows);
+    }
+    if (items >= target || rows <= 8) return rows;
+    rows = rows / 2 < 8 ? 8 : rows / 2;
+  }
+}
 
 // One tick over the active jobs; returns the cell-iterations done.
 double tick(hmg_server *sv, std::vector<Job *> &act)
@@ -131,26 +170,34 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
   int rc = HMG_OK;
   if (!vs.empty())
   {
-    const bool probing = sv->probe < 2 * N_SERVE_CAND;
-    const int  ci = sv->probe % N_SERVE_CAND;
-    const int  rows = probing ? SERVE_CAND[ci][0] : sv->rows;
-    const int  variant = probing ? SERVE_CAND[ci][1] : sv->variant;
-    float      ms = 0.f;
+    double cells = 0, tick_cells = 0;
+    bool   young = false;
+    for (size_t q = 0; q < vj.size(); q++)
+    {
+      const double c = (double)vj[q]->desc.nx * vj[q]->desc.ny;
+      tick_cells += c;
+      cells += c * vi[q];
+      young = young || vj[q]->done < TUNE_AFTER_IT;
+    }
+    hmg_server::Shape &sh = sv->shapes[(int)std::floor(std::log2(tick_cells))];
+    const bool         probing = !young && sh.probe < 2 * N_SERVE_CAND;
+    const int          ci = sh.probe % N_SERVE_CAND;
+    const int          rows = capped_rows(sv, vj, probing ? SERVE_CAND[ci][0] : sh.rows);
+    const int          variant = probing ? SERVE_CAND[ci][1] : sh.variant;
+    float              ms = 0.f;
     rc = batch_advance(ctx, vs.data(), vi.data(), (int)vs.size(), 0, rows, variant,
                        probing ? &ms : nullptr);
     if (probing && rc == HMG_OK)
     {
-      double cells = 0;
-      for (size_t q = 0; q < vj.size(); q++) cells += (double)vj[q]->desc.nx * vj[q]->desc.ny * vi[q];
       const double c = cells > 0 ? ms / cells : 0.0;
-      if (sv->probe < N_SERVE_CAND || c < sv->cost[ci]) sv->cost[ci] = c;
-      if (++sv->probe == 2 * N_SERVE_CAND)
+      if (sh.probe < N_SERVE_CAND || c < sh.cost[ci]) sh.cost[ci] = c;
+      if (++sh.probe == 2 * N_SERVE_CAND)
       {
         int b = 0;
         for (int q = 1; q < N_SERVE_CAND; q++)
-          if (sv->cost[q] < sv->cost[b]) b = q;
-        sv->rows = SERVE_CAND[b][0];
-        sv->variant = SERVE_CAND[b][1];
+          if (sh.cost[q] < sh.cost[b]) b = q;
+        sh.rows = SERVE_CAND[b][0];
+        sh.variant = SERVE_CAND[b][1];
       }
     }
     for (size_t q = 0; q < vj.size(); q++)
@@ -158,7 +205,7 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
       Job *j = vj[q];
       if (rc != HMG_OK)
       {
-        j->state = rc;
+        set_state(sv, j, rc);
         continue;
       }
       j->done += vi[q];
@@ -167,7 +214,7 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
       if (every > 0 && j->done % every == 0)
       {
         int r2 = hmg_session_thermal(j->s, j->desc.talus, 1);
-        if (r2 != HMG_OK) j->state = r2;
+        if (r2 != HMG_OK) set_state(sv, j, r2);
       }
     }
   }
@@ -182,7 +229,7 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
                                        nullptr, j->desc.talus, k, nullptr, ctx->stream, &res);
       if (r2 != HMG_OK)
       {
-        j->state = r2;
+        set_state(sv, j, r2);
         continue;
       }
       // an odd number of launches leaves the result in the other plane
@@ -192,7 +239,7 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
       if (j->done == j->desc.iterations)
       {
         r2 = launch_extrapolate_borders(s->z[s->cur].as<float>(), s->nx, s->ny, ctx->stream);
-        if (r2 != HMG_OK) j->state = r2;
+        if (r2 != HMG_OK) set_state(sv, j, r2);
       }
     }
   (void)hipStreamSynchronize(ctx->stream);
@@ -229,10 +276,11 @@ void worker_main(hmg_server *sv)
     {
       if (j->bytes > sv->budget)
       {
-        j->state = HMG_ERR_OOM;
+        set_state(sv, j, HMG_ERR_OOM);
         continue;
       }
-      int rc = admit(sv, j);
+      int                         rc = admit(sv, j);
+      std::lock_guard<std::mutex> lk(sv->mu);
       j->t_admit = now_s();
       j->state = rc == HMG_OK ? HMG_JOB_RUNNING : rc;
     }
@@ -246,29 +294,37 @@ void worker_main(hmg_server *sv)
     const double work = act.empty() ? 0.0 : tick(sv, act);
     // results of finished (or failed) jobs to the callers' host buffers
     std::vector<Job *> finished;
+    std::vector<int>   final_state;
     for (Job *j : sv->active)
       if (j->state != HMG_JOB_RUNNING || j->done >= j->desc.iterations) finished.push_back(j);
     for (Job *j : finished)
     {
-      if (j->state == HMG_JOB_RUNNING && j->s)
+      int st = j->state;
+      if (st == HMG_JOB_RUNNING && j->s)
       {
         int rc = HMG_OK;
         // thermal() extrapolates the borders even after zero steps
         if (j->desc.kind == HMG_JOB_THERMAL && j->desc.iterations == 0)
           rc = launch_extrapolate_borders(j->s->z[j->s->cur].as<float>(), j->s->nx, j->s->ny,
                                           sv->ctx->stream);
+        // the export checks the state: a job whose state holds a NaN or an
+        // infinity fails with HMG_ERR_NONFINITE, alone
         if (rc == HMG_OK)
           rc = hmg_session_export(j->s, j->out, nullptr, nullptr, nullptr, nullptr);
-        j->state = rc == HMG_OK ? HMG_JOB_DONE : rc;
+        st = rc == HMG_OK ? HMG_JOB_DONE : rc;
       }
       if (j->s) hmg_session_destroy(j->s);
       j->s = nullptr;
-      j->t_done = now_s();
+      final_state.push_back(st);
     }
     {
       std::lock_guard<std::mutex> lk(sv->mu);
-      for (Job *j : finished)
+      const double                t_done = now_s();
+      for (size_t q = 0; q < finished.size(); q++)
       {
+        Job *j = finished[q];
+        j->state = final_state[q];
+        j->t_done = t_done;
         sv->used -= j->bytes;
         sv->active_cells -= (size_t)j->desc.nx * j->desc.ny;
         sv->active.erase(std::find(sv->active.begin(), sv->active.end(), j));

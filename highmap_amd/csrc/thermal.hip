@@ -512,7 +512,8 @@ static int thermal_host_call(hmg_ctx     *ctx,
   HMG_HIP(hipMemcpyAsync(z_host, zb.ptr, bytes, hipMemcpyDeviceToHost, st));
   if (pd) HMG_HIP(hipMemcpyAsync(deposition_host, pd, bytes, hipMemcpyDeviceToHost, st));
   HMG_HIP(hipStreamSynchronize(st));
-  return HMG_OK;
+  // the result is delivered either way; a non-finite one is reported
+  return check_finite_f32(ctx, zb.as<float>(), n);
 }
 
 int hmg_thermal(hmg_ctx     *ctx,

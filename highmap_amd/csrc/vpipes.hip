@@ -1324,6 +1324,96 @@ static int session_create_common(hmg_ctx                 *ctx,
   return HMG_OK;
 }
 
+// ------------------------------------------------------------ finite check
+//
+// A NaN or an infinity in a state (a non-finite input cell, for one) would
+// otherwise come back as an ordinary result: a NaN height stays NaN through
+// every pass while max/min clamps keep its neighbours finite.  Export, the
+// sums and the server's job completion therefore scan the state once.  A
+// word is non-finite when its exponent bits are all ones (FP32 0x7f800000,
+// BF16 0x7f80, FP16 0x7c00).  The flag only goes from 0 to 1, by plain
+// stores of the same value.
+
+__global__ void k_flag_clear(unsigned int *f)
+{
+  *f = 0u;
+}
+
+template <class W>
+__global__ void __launch_bounds__(256) k_nonfinite(const W *__restrict__ a,
+                                                   size_t n,
+                                                   W      mask,
+                                                   unsigned int *flag)
+{
+  bool bad = false;
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    bad |= (a[c] & mask) == mask;
+  if (bad) *flag = 1u;
+}
+
+template <class W>
+static int launch_nonfinite(const void *p, size_t n, W mask, unsigned int *flag, hipStream_t st)
+{
+  const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  if (nb > 0) k_nonfinite<W><<<nb, 256, 0, st>>>((const W *)p, n, mask, flag);
+  return check_launch("nonfinite");
+}
+
+// HMG_ERR_NONFINITE if a float plane of n values on the context's device
+// holds a NaN or an infinity.  Waits for the context stream.
+int check_finite_f32(hmg_ctx *ctx, const float *p, size_t n)
+{
+  hipStream_t st = ctx->stream;
+  DevBuf      flag;
+  HMG_TRY(flag.get(ctx->pool, sizeof(unsigned int)));
+  unsigned int *f = flag.as<unsigned int>();
+  k_flag_clear<<<1, 1, 0, st>>>(f);
+  HMG_TRY(check_launch("flag_clear"));
+  HMG_TRY(launch_nonfinite<uint32_t>(p, n, 0x7f800000u, f, st));
+  unsigned int h = 0;
+  HMG_HIP(hipMemcpyAsync(&h, f, sizeof(h), hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  if (h) return set_error(HMG_ERR_NONFINITE, "the result holds a NaN or an infinity");
+  return HMG_OK;
+}
+
+// HMG_ERR_NONFINITE if z, the water, the sediment or a flux of the
+// session's current state holds a NaN or an infinity.  Waits for the stream.
+int session_check_finite(hmg_session *s)
+{
+  const size_t n = (size_t)s->nx * s->ny;
+  hipStream_t  st = s->ctx->stream;
+  DevBuf       flag;
+  HMG_TRY(flag.get(s->ctx->pool, sizeof(unsigned int)));
+  unsigned int *f = flag.as<unsigned int>();
+  k_flag_clear<<<1, 1, 0, st>>>(f);
+  HMG_TRY(check_launch("flag_clear"));
+  const int c = s->cur;
+  HMG_TRY(launch_nonfinite<uint32_t>(s->z[c].ptr, n, 0x7f800000u, f, st));
+  if (s->storage == HMG_STORAGE_F32)
+  {
+    HMG_TRY(launch_nonfinite<uint32_t>(s->D[c].ptr, n, 0x7f800000u, f, st));
+    HMG_TRY(launch_nonfinite<uint32_t>(s->Sp[c].ptr, n, 0x7f800000u, f, st));
+    HMG_TRY(launch_nonfinite<uint32_t>(s->F[c].ptr, 4 * n, 0x7f800000u, f, st));
+  }
+  else
+  {
+    const uint16_t m = s->storage == HMG_STORAGE_BF16 ? 0x7f80u : 0x7c00u;
+    HMG_TRY(launch_nonfinite<uint16_t>(s->D[c].ptr, n, m, f, st));
+    HMG_TRY(launch_nonfinite<uint16_t>(s->Sp[c].ptr, n, m, f, st));
+    HMG_TRY(launch_nonfinite<uint16_t>(s->F[c].ptr, 4 * n, m, f, st));
+  }
+  unsigned int h = 0;
+  HMG_HIP(hipMemcpyAsync(&h, f, sizeof(h), hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  if (h)
+    return set_error(HMG_ERR_NONFINITE,
+                     "the %dx%d state holds a NaN or an infinity after %ld iterations", s->nx,
+                     s->ny, s->iter);
+  return HMG_OK;
+}
+
 } // namespace hmg
 
 extern "C" {
@@ -1419,7 +1509,8 @@ int hmg_session_export(hmg_session *s, float *z_host, float *d_host, float *s_ho
     HMG_HIP(hipStreamSynchronize(st));
   }
   HMG_HIP(hipStreamSynchronize(st));
-  return HMG_OK;
+  // the planes are delivered either way; a non-finite state is reported
+  return session_check_finite(s);
 }
 
 // max(|u|, |v|) over n cells as the bit pattern of a non-negative float: for
@@ -1532,7 +1623,8 @@ int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d
     }
     if (dst[q]) *dst[q] = fix_to_double(a.lo, a.hi);
   }
-  return HMG_OK;
+  // a fixed-point sum of non-finite values is meaningless
+  return session_check_finite(s);
 }
 
 int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch)

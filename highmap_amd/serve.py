@@ -162,7 +162,7 @@ def bench(args):
     import highmap_amd as hm
 
     hm.init_gpu(0)
-    storage = Storage.BF16 if args.storage == "f32" else {"bf16": 1, "f16": 2}[args.storage]
+    storage = {"f32": Storage.F32, "bf16": Storage.BF16, "f16": Storage.F16}[args.storage]
     params = VpipesParams.make(water_height=1e-3, evap_rate=0.01, dt=0.5, k_capacity=0.1,
                                k_erode=0.05, k_depose=0.05)
     reqs = workloads.serve_requests(n=args.requests)
