@@ -143,6 +143,10 @@ int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
 // infinity; waits for the context stream (vpipes.hip).
 int check_finite_f32(hmg_ctx *ctx, const float *p, size_t n);
 
+// Looks up, once per device, the device address of the static scratch plane
+// of hmg_thermal_dev_noalloc (thermal.hip); called when a context is created.
+int noalloc_scratch_setup(int device);
+
 // Per-block min and max of a[0..n) into mm[2 + 2b], mm[3 + 2b] for the nblk
 // blocks of the launch (nblk <= 1024; mm holds 2 + 2 * nblk floats).
 int launch_minmax(const float *a, size_t n, float *mm, int nblk, hipStream_t st);

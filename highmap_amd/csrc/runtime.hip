@@ -203,6 +203,7 @@ int hmg_init(int device, hmg_ctx **out)
   if (device < 0 || device >= n)
     return set_error(HMG_ERR_INVALID, "device %d out of range [0, %d)", device, n);
   HMG_HIP(hipSetDevice(device));
+  HMG_TRY(noalloc_scratch_setup(device));
   hipDeviceProp_t prop;
   HMG_HIP(hipGetDeviceProperties(&prop, device));
   hmg_ctx *c = new hmg_ctx();

@@ -22,6 +22,7 @@
 //  k_extrap_cols / k_extrap_rows  extrapolate_borders, columns then rows.
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
 
 #include "hmg_internal.hpp"
 
@@ -389,6 +390,40 @@ int launch_thermal_auto(float       *z,
   return HMG_OK;
 }
 
+// ------------------------------------------------- allocation-free scratch
+//
+// hmg_thermal_dev_noalloc's second Jacobi plane: a static device array, one
+// per GPU the library runs on, held until the process ends.  Host code never
+// passes `scratch` itself to a kernel (on the host it names a stand-in, not
+// device memory): its device address is looked up once per device, with that
+// device current, when a context is created on it, and kept in g_scratch_ptr.
+constexpr size_t NOALLOC_MAX_CELLS = (size_t)4096 * 4096; // 64 MiB of float
+static __device__ __align__(256) float scratch[NOALLOC_MAX_CELLS];
+
+constexpr int     MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static float     *g_scratch_ptr[MAX_DEVICES] = {};
+
+int noalloc_scratch_setup(int device)
+{
+  if (device < 0 || device >= MAX_DEVICES)
+    return set_error(HMG_ERR_INVALID, "device %d beyond the scratch table", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return HMG_OK;
+  HMG_HIP(hipSetDevice(device));
+  void *p = nullptr;
+  HMG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<float *>(p);
+  return HMG_OK;
+}
+
+static float *noalloc_scratch(int device)
+{
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 static int post_blocks(size_t n)
 {
   return (int)std::min<size_t>((n + 255) / 256, 4096);
@@ -427,6 +462,7 @@ int hmg_thermal_dev(hmg_ctx     *ctx,
 {
   clear_error();
   HMG_TRY(check_thermal_args(nx, ny, iterations));
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
   if (!z_dev) return set_error(HMG_ERR_INVALID, "null z");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   HMG_TRY(ctx_enter(ctx));
@@ -438,6 +474,34 @@ int hmg_thermal_dev(hmg_ctx     *ctx,
   // the scratch plane returns to the pool when this call ends
   HMG_HIP(hipStreamSynchronize(ctx->stream));
   return HMG_OK;
+}
+
+int hmg_thermal_dev_noalloc(hmg_ctx     *ctx,
+                            float       *z_dev,
+                            int          nx,
+                            int          ny,
+                            const float *talus_dev,
+                            float        talus_scalar,
+                            int          iterations,
+                            const float *bedrock_dev)
+{
+  clear_error();
+  HMG_TRY(check_thermal_args(nx, ny, iterations));
+  if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
+  if (!z_dev) return set_error(HMG_ERR_INVALID, "null z");
+  if ((size_t)nx * (size_t)ny > NOALLOC_MAX_CELLS)
+    return set_error(HMG_ERR_INVALID,
+                     "hmg_thermal_dev_noalloc: %d x %d is above its %zu-cell scratch plane", nx,
+                     ny, (size_t)NOALLOC_MAX_CELLS);
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  HMG_TRY(ctx_enter(ctx));
+  float *zs = noalloc_scratch(ctx->device);
+  if (!zs) return set_error(HMG_ERR_DEVICE, "no scratch plane on device %d", ctx->device);
+  // the Jacobi ping-pong runs through the device's static plane; an odd
+  // number of launches copies the result back into z_dev (a device copy)
+  HMG_TRY(launch_thermal(ctx, z_dev, zs, nx, ny, talus_dev, talus_scalar, iterations,
+                         bedrock_dev, ctx->stream));
+  return launch_extrapolate_borders(z_dev, nx, ny, ctx->stream);
 }
 
 // Host-pointer form shared by thermal and thermal_auto_bedrock: upload,

@@ -221,8 +221,9 @@ int hmg_thermal(hmg_ctx     *ctx,
                 float       *deposition_host,
                 const float *mask_host);
 /* Device form: `iterations` Jacobi steps then extrapolate_borders, in place
- * on z_dev, asynchronous on the context stream.  talus_dev / bedrock_dev may
- * be NULL. */
+ * on z_dev.  The second Jacobi plane comes from the context's buffer pool
+ * and goes back to it when the call returns, so the call waits for its work
+ * on the context stream.  talus_dev / bedrock_dev may be NULL. */
 int hmg_thermal_dev(hmg_ctx     *ctx,
                     float       *z_dev,
                     int          nx,
@@ -231,6 +232,23 @@ int hmg_thermal_dev(hmg_ctx     *ctx,
                     float        talus_scalar,
                     int          iterations,
                     const float *bedrock_dev);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same
+ * computation as hmg_thermal_dev, with the second Jacobi plane in a static
+ * device array of 4096 x 4096 floats (64 MiB) that each GPU holds for the
+ * life of the process; it calls no hipMalloc/hipFree and allocates nothing.
+ * Grids above 4096 x 4096 cells are HMG_ERR_INVALID.  Asynchronous on the
+ * context stream.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (from different contexts or streams) race and give
+ * wrong results; calls on one context are ordered by its stream. */
+int hmg_thermal_dev_noalloc(hmg_ctx     *ctx,
+                            float       *z_dev,
+                            int          nx,
+                            int          ny,
+                            const float *talus_dev,
+                            float        talus_scalar,
+                            int          iterations,
+                            const float *bedrock_dev);
 
 /* hmap::gpu::thermal_auto_bedrock, the three overloads
  * (HighMap/src/erosion/thermal_gpu.cpp:98-146, kernel thermal.cl:102-145;
