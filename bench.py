@@ -166,10 +166,19 @@ def run_headline(args):
     launch_s = e0.elapsed_time(e1) / 1e3 / m
     alg = ALG_BYTES[args.storage] * n * n
     achieved = alg / launch_s / 1e9
+    # the thermal step alone (one streaming launch with the borders folded
+    # in), on the same state
+    m_th = 20
+    e0.record(stream)
+    for _ in range(m_th):
+        sess.thermal(talus, 1)
+    e1.record(stream)
+    e1.synchronize()
+    thermal_us = e0.elapsed_time(e1) * 1e3 / m_th
     chunk_rows, prefetch, _ = sess.tuning()
     max_backtrace = sess.max_backtrace()
     sess.close()
-    copy_gbs = measured_copy_gbs(torch)
+    copy_gbs, copy_variant = measured_copy_gbs(torch)
 
     kernel = (("k_vp_stream_ga" if prefetch == 2 else "k_vp_stream") if args.mode == "fused"
               else "k_vp_* (plain passes)")
@@ -201,6 +210,7 @@ def run_headline(args):
         # how far (cells) the last state's sediment backtraces reach: the
         # gathers of this workload land hundreds of cells away
         "max_backtrace_cells": round(max_backtrace, 2),
+        "thermal_step_us": round(thermal_us, 2),
         "pct_hbm_peak_step": round(100.0 * (ALG_BYTES[args.storage] * n * n * iters +
                                             8 * n * n * args.steps) / elapsed / 1e9
                                    / HBM_PEAK_GBS, 2),
@@ -221,6 +231,8 @@ def run_headline(args):
     # on this GPU in this process: the practical ceiling for this mix
     tr = out["roofline"]["traffic"]
     out["measured_copy"] = {"copy_hbm_gbs": round(copy_gbs, 1),
+                            "copy_kernel": dict(copy_variant, source="hbm_probe.hip copy, "
+                                                "fastest variant, 2 GiB buffers"),
                             "traffic_gbs": round(tr / launch_s / 1e9, 1) if tr else None,
                             "traffic_frac_of_copy": round(tr / launch_s / 1e9 / copy_gbs, 4)
                             if tr else None,
@@ -411,8 +423,8 @@ def run_thermal256(args):
                                            50, None))
     dev_s = (time.perf_counter() - t0) / reps
     o64 = O.thermal(z0.astype(np.float64), 2.0 / 256, 50, np.float64)
-    # the same call at 4096^2, device-resident (LDS-blocked kernel, 8 steps
-    # per launch): events around the call, best of 3
+    # the same call at 4096^2, device-resident (the streaming kernel, one
+    # step per launch): events around the call, best of 3
     big = 4096
     zb = device_terrain(torch, hm, ctx, big, big, 1)
     st = torch.cuda.ExternalStream(ctx.stream)
@@ -644,26 +656,73 @@ def run_vpipes1024(args):
             "cpu_all_threads_gcell_iter_s": c["gcell_iter_s"], "cpu_threads": c["threads"]}
 
 
-def measured_copy_gbs(torch, mib=1024):
-    """Read + write bytes per second of the best of three device copies of
-    `mib` MiB (caches flushed by a fill of twice the size in between)."""
-    n = mib * 256 * 1024
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
+_PROBE = None
+
+
+def probe_lib():
+    """libhmg_probe.so: the HBM stream kernels (read, write, copy)."""
+    global _PROBE
+    if _PROBE is None:
+        import ctypes as C
+
+        path = os.path.join(ROOT, "highmap_amd", "libhmg_probe.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libhmg_probe.so not built (python -m highmap_amd.build)")
+        L = C.CDLL(path)
+        L.hmg_probe_stream.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                       C.POINTER(C.c_float)]
+        L.hmg_probe_stream.restype = C.c_int
+        _PROBE = L
+    return _PROBE
+
+
+STREAM_MODES = {"read": 0, "write": 1, "copy": 2}
+
+
+def stream_sweep(torch, gib=4, modes=("read", "write", "copy"), unrolls=(1, 2, 4, 8),
+                 nts=(0, 1), blocks_per_cu=(0, 4, 8, 16), reps=3):
+    """Times the HBM stream kernels over `gib` GiB buffers (far beyond the
+    256 MB of cache) for every variant; returns {mode: [(GB/s, variant)...]}
+    sorted fastest first.  Bytes per launch: read gib, write gib, copy 2 gib."""
+    import ctypes as C
+
+    L = probe_lib()
+    nbytes = gib << 30
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
-    flush = torch.empty(2 * n, dtype=torch.float32, device="cuda")
-    best = None
-    for _ in range(3):
-        flush.fill_(1.0)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        b.copy_(a)
-        e1.record()
-        e1.synchronize()
-        ms = e0.elapsed_time(e1)
-        best = ms if best is None else min(best, ms)
-    del a, b, flush
+    sink = torch.zeros(1, dtype=torch.float32, device="cuda")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    torch.cuda.synchronize()
+    out = {}
+    for mode in modes:
+        rows = []
+        moved = nbytes * (2 if mode == "copy" else 1)
+        for u in unrolls:
+            for nt in nts:
+                for bpc in blocks_per_cu:
+                    ms = C.c_float()
+                    rc = L.hmg_probe_stream(STREAM_MODES[mode], a.data_ptr(), b.data_ptr(),
+                                            nbytes, u, nt, bpc * cus, reps, sink.data_ptr(),
+                                            C.byref(ms))
+                    if rc != 0:
+                        raise RuntimeError("hmg_probe_stream failed (%d)" % rc)
+                    rows.append((round(moved / (ms.value / 1e3) / 1e9, 1),
+                                 {"unroll": u, "nontemporal": nt,
+                                  "grid": "one pass" if bpc == 0 else "%d blocks/CU" % bpc}))
+        rows.sort(key=lambda r: -r[0])
+        out[mode] = rows
+    del a, b
     torch.cuda.empty_cache()
-    return 8 * n / (best / 1e3) / 1e9
+    return out
+
+
+def measured_copy_gbs(torch):
+    """Read + write bytes per second of the fastest HBM copy kernel variant
+    (2 GiB buffers), and that variant."""
+    rows = stream_sweep(torch, gib=2, modes=("copy",), unrolls=(2, 4, 8),
+                        blocks_per_cu=(0, 8), reps=3)["copy"]
+    return rows[0]
 
 
 def run_copy(args):
@@ -706,6 +765,25 @@ def run_copy(args):
             "wall_s": round(wall, 4)}
 
 
+def run_ceiling(args):
+    """What this GPU's HBM delivers to streaming kernels: read-only,
+    write-only and copy over 4 GiB buffers, every variant of
+    highmap_amd/csrc/hbm_probe.hip (1-8 float4 loads in flight per lane,
+    nontemporal or not, one-pass or striding grids), best of 3 each."""
+    import torch
+
+    t0 = time.perf_counter()
+    res = stream_sweep(torch, gib=4)
+    out = {"metric": "HBM stream ceiling", "unit": "GB/s", "n_gpus": 1,
+           "peak_gbs": HBM_PEAK_GBS, "buffer_gib": 4}
+    for mode, rows in res.items():
+        out[mode] = {"best_gbs": rows[0][0], "best_variant": rows[0][1],
+                     "frac_of_peak": round(rows[0][0] / HBM_PEAK_GBS, 4),
+                     "all": [[g, v["unroll"], v["nontemporal"], v["grid"]] for g, v in rows]}
+    out["wall_s"] = round(time.perf_counter() - t0, 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -726,7 +804,7 @@ def main():
                     help="serve: cells of the jobs that run at once (0: no cap)")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batch", "serve", "precision", "thermal256",
-                             "vpipes1024", "flow", "snow", "tiles", "copy"])
+                             "vpipes1024", "flow", "snow", "tiles", "copy", "ceiling"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -738,7 +816,7 @@ def main():
         sys.exit(2)
     fn = {"headline": run_headline, "batch": run_batch, "precision": run_precision,
           "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "flow": run_flow,
-          "snow": run_snow, "tiles": run_tiles, "copy": run_copy}
+          "snow": run_snow, "tiles": run_tiles, "copy": run_copy, "ceiling": run_ceiling}
     if args.workload == "serve":
         from highmap_amd import serve
 

@@ -132,6 +132,7 @@ def lib():
             "hmg_thermal": ([vp, vp, i32, i32, vp, f32, i32, vp, vp, vp], i32),
             "hmg_thermal_dev": ([vp, vp, i32, i32, vp, f32, i32, vp], i32),
             "hmg_thermal_dev_noalloc": ([vp, vp, i32, i32, vp, f32, i32, vp], i32),
+            "hmg_thermal_set_block_steps": ([i32], i32),
             "hmg_vpipes": ([vp, vp, i32, i32, pp, vp, vp, vp, vp, vp], i32),
             "hmg_session_create": ([vp, vp, i32, i32, pp, vp, i32, C.POINTER(vp)], i32),
             "hmg_session_create_dev": ([vp, vp, i32, i32, pp, vp, i32, C.POINTER(vp)], i32),

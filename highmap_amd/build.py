@@ -22,6 +22,8 @@ CSRC = os.path.join(HERE, "csrc")
 INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhmg.so")
+# HBM stream kernels used only by bench.py's measurements
+PROBE_LIB = os.path.join(HERE, "libhmg_probe.so")
 SHIM_LIB = os.path.join(HERE, "libhmap_gpu.so")
 ARCH = os.environ.get("HMG_OFFLOAD_ARCH", "gfx950")
 
@@ -75,6 +77,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([_hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    probe = os.path.join(CSRC, "hbm_probe.hip")
+    if os.path.exists(probe):
+        tmp = PROBE_LIB + ".tmp"
+        _run([_hipcc()] + HIPCC_FLAGS + ["-shared", "-x", "hip", probe, "-o", tmp])
+        os.replace(tmp, PROBE_LIB)
     shim = os.path.join(CSRC, "hmap_gpu.cpp")
     if os.path.exists(shim):
         tmp = SHIM_LIB + ".tmp"

@@ -125,7 +125,9 @@ int validate_vp_params(const hmg_vpipes_params &p);
 // Runs `iterations` Jacobi thermal steps on z, ping-ponging with zscratch.
 // With result == nullptr the result ends in z (copied back after an odd
 // number of launches); otherwise *result receives whichever of the two
-// planes holds it and nothing is copied.
+// planes holds it and nothing is copied.  With extrapolate, the steps are
+// followed by extrapolate_borders (folded into the last step's launch where
+// the streaming kernel runs it).
 int launch_thermal(hmg_ctx     *ctx,
                    float       *z,
                    float       *zscratch,
@@ -136,7 +138,8 @@ int launch_thermal(hmg_ctx     *ctx,
                    int          iterations,
                    const float *bedrock,
                    hipStream_t  st,
-                   float      **result = nullptr);
+                   float      **result = nullptr,
+                   bool         extrapolate = false);
 int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
 
 // HMG_ERR_NONFINITE if the n floats at p (device) hold a NaN or an

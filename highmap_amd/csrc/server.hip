@@ -225,8 +225,11 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
       if (k > 8) k = 8;
       hmg_session *s = j->s;
       float       *zc = s->z[s->cur].as<float>(), *res = zc;
+      // the job's last steps end with thermal()'s extrapolate_borders
+      const bool   last = j->done + k == j->desc.iterations;
       int          r2 = launch_thermal(ctx, zc, s->z[s->cur ^ 1].as<float>(), s->nx, s->ny,
-                                       nullptr, j->desc.talus, k, nullptr, ctx->stream, &res);
+                                       nullptr, j->desc.talus, k, nullptr, ctx->stream, &res,
+                                       last);
       if (r2 != HMG_OK)
       {
         set_state(sv, j, r2);
@@ -236,11 +239,6 @@ double tick(hmg_server *sv, std::vector<Job *> &act)
       if (res != zc) std::swap(s->z[s->cur].ptr, s->z[s->cur ^ 1].ptr);
       j->done += k;
       work += (double)j->desc.nx * j->desc.ny * k;
-      if (j->done == j->desc.iterations)
-      {
-        r2 = launch_extrapolate_borders(s->z[s->cur].as<float>(), s->nx, s->ny, ctx->stream);
-        if (r2 != HMG_OK) set_state(sv, j, r2);
-      }
     }
   (void)hipStreamSynchronize(ctx->stream);
   return work;

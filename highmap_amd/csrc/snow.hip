@@ -299,8 +299,7 @@ static int snow_run(hmg_ctx     *ctx,
   HMG_HIP(hipMemcpyAsync(zf.ptr, zw.ptr, bytes, hipMemcpyDeviceToDevice, st));
   // the thermal scratch plane: `other` is free again
   HMG_TRY(launch_thermal(ctx, zf.as<float>(), other.as<float>(), nx, ny, tal.as<float>(), 0.f,
-                         iterations, nullptr, st));
-  HMG_TRY(launch_extrapolate_borders(zf.as<float>(), nx, ny, st));
+                         iterations, nullptr, st, nullptr, true));
   k_snow_finish<<<flat_blocks(n), 256, 0, st>>>(s, z, zw.as<float>(), zf.as<float>(),
                                                 mask.as<float>(), n);
   HMG_TRY(check_launch("snow_finish"));

@@ -12,14 +12,18 @@
 // is Jacobi: step t is read, step t+1 is written to the other buffer.
 //
 // Kernels
-//  k_thermal_step    one Jacobi step, one cell per lane, 9-point stencil
-//                    served from L1/LDS-backed cache lines.  HBM-bound:
-//                    8 B/cell (z read + write), 12 B/cell with a talus plane.
+//  k_thermal_stream  one Jacobi step with a uniform talus: wavefronts stream
+//                    strips of 62 columns down the rows, each exchange formed
+//                    once for the pair of cells it joins; the last step of a
+//                    thermal() call also applies extrapolate_borders.
+//                    8 B/cell (z read + write).
+//  k_thermal_step    one Jacobi step, one cell per lane, 9-point stencil, for
+//                    a talus plane (12 B/cell) or thermal_with_bedrock.
 //  k_thermal_blocked T Jacobi steps per launch on an LDS tile with a T-cell
-//                    halo (temporal blocking): the tile is read once and
-//                    written once per T steps, so HBM traffic falls to
-//                    ~8/T B per cell-step plus the halo overhead.
-//  k_extrap_cols / k_extrap_rows  extrapolate_borders, columns then rows.
+//                    halo (temporal blocking), for grids below 2^18 cells
+//                    where launches dominate.
+//  k_extrap_cols / k_extrap_rows  extrapolate_borders, columns then rows,
+//                    where it is not folded into the last step.
 #include <algorithm>
 #include <cfloat>
 #include <mutex>
@@ -289,12 +293,267 @@ int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st)
   return check_launch("extrapolate_borders(rows)");
 }
 
-// Temporal-blocking depth used for uniform talus; 0 selects one step per
-// launch.  Exposed for the tuner.
+// ------------------------------------------------------ streaming step
+//
+// One Jacobi step with a uniform talus, one launch.  Each wavefront owns a
+// strip of 64 columns (62 written, one halo column each side) and a chunk
+// of rows, and streams down the rows with the heights in a register ring
+// (rows loaded P rows ahead, one load per cell).
+//
+// The exchange between two cells is antisymmetric: with d = a - b, the
+// reference's helper_thermal_exchange gives E(a, b) = -E(b, a) exactly
+// (|d| and the threshold are the same from both sides, IEEE negation and
+// subtraction are exact).  So each cell forms only the exchanges with its
+// right (i+1, j), upper (i, j+1), upper-right and upper-left neighbours,
+// and takes the other four, negated, from the lane to its left or right
+// (DPP) or from the previous row's registers.  That halves the work of the
+// step.  The eight terms are then added in the reference's order
+// (-1,0),(0,1),(0,-1),(1,0),(-1,-1),(-1,1),(1,-1),(1,1), a negated term as
+// a subtraction.
+//
+// One exchange is copysign(m, other - self) with
+// m = max(0.2 (|other - self| - c T), 0) (divided by 1.414 on diagonals):
+// m > 0 exactly when |d| > c T (denormals kept, so a - b > 0 iff a > b), and
+// where the reference adds +0 this may add -0, which leaves the running sum
+// unchanged (the sum starts as 0 - term and is never -0).  Talus must be
+// >= 0 (the host checks; other values take k_thermal_step).
+//
+// EXTRAP: the step is the last of a thermal() call, and extrapolate_borders
+// (boundary.cpp:19-41) is applied in the same launch: border columns of the
+// interior rows from the two cells inward (lanes to the right or left), then
+// rows 0 and ny - 1 from the two rows inward (kept in registers; row 0 is
+// written when row 2 is done).  Needs nx, ny >= 4; a chunk that holds row
+// ny - 1 starts at ny - 3 at the latest, and chunks are >= 3 rows, so both
+// row pairs are in one wavefront's registers.  Where chunks overlap or the
+// last strip is moved left to end at column nx - 1, two wavefronts write the
+// same values.
+
+constexpr int TH_OUT = 62; // columns written per wavefront
+
+__device__ __forceinline__ float th_left(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float th_right(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+// E(self, other) of an axial neighbour (c = 1).
+__device__ __forceinline__ float th_pair_axial(float self, float other, float md)
+{
+  const float nd = other - self;
+  const float m = fmaxf(0.2f * (fabsf(nd) - md), 0.f);
+  return copysignf(m, nd);
+}
+
+// E(self, other) of a diagonal neighbour (c = 1.414): m / 1.414 through
+// div_1414's product and fma correction, which is the correctly rounded
+// quotient for m = 0 and every m in [2^-100, inf).  `odd` is set where m is
+// outside those (tiny or infinite); the caller then recomputes the row's
+// diagonal exchanges with the IEEE division.
+template <bool IEEE>
+__device__ __forceinline__ float th_pair_diag(float self, float other, float mdd, bool &odd)
+{
+  constexpr float B = 1.414f;
+  constexpr float R = 0x1.6a17e6p-1f; // RN(1 / 1.414f)
+  const float     nd = other - self;
+  const float     m = fmaxf(0.2f * (fabsf(nd) - mdd), 0.f);
+  float           q;
+  if (IEEE)
+    q = m / B;
+  else
+  {
+    const uint32_t b = __float_as_uint(m);
+    odd = odd || (b - 1u < 0x0d7fffffu) || b >= 0x7f800000u; // (0, 2^-100) or inf
+    const float q0 = m * R;
+    q = fmaf(fmaf(-q0, B, m), R, q0);
+  }
+  return copysignf(q, nd);
+}
+
+__device__ __forceinline__ int th_wave_item()
+{
+  const int G = gridDim.x;
+  const int b = (G & 7) ? blockIdx.x : (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  return __builtin_amdgcn_readfirstlane(b * 4 + (threadIdx.x >> 6));
+}
+
+template <int P, bool EXTRAP>
+__global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict__ z,
+                                                        float *__restrict__ zn,
+                                                        int   nx,
+                                                        int   ny,
+                                                        float ts,
+                                                        int   strips,
+                                                        int   chunk_rows,
+                                                        int   n_items)
+{
+  const int item = th_wave_item();
+  if (item >= n_items) return;
+  const int lane = threadIdx.x & 63;
+  const int strip = item % strips, chunk = item / strips;
+  // the last strip ends at column nx - 1 (its columns overlap the previous
+  // strip's; both write the same values)
+  const int      x0 = nx > TH_OUT ? min(strip * TH_OUT - 1, nx - TH_OUT - 1) : -1;
+  const int      gx = x0 + lane;
+  const uint32_t off = (uint32_t)min(max(gx, 0), nx - 1); // column loaded
+  const uint32_t soff = (uint32_t)max(gx, 0);              // column stored
+  const bool     writer = lane >= 1 && lane <= TH_OUT && gx < nx;
+  const bool     c0 = gx == 0, cn = gx == nx - 1;
+  // strips that hold column 0 or nx - 1 take the border rules in every row
+  const bool edge_strip = x0 < 0 || x0 + TH_OUT + 1 >= nx;
+  int        j0 = chunk * chunk_rows;
+  const int  j1 = min(j0 + chunk_rows, ny);
+  if (EXTRAP && j1 - j0 < 3) j0 = j1 - 3;
+  const float md = ts, mdd = 1.414f * ts;
+
+  // a row's base in scalar registers, the column as a 32-bit lane offset
+  auto load = [&](int r) {
+    const float *row = z + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
+    return at_off(row, off);
+  };
+
+  constexpr int U = P + 3; // rows r-1 .. r+P+1
+  float         R[U];
+  // pre-roll: rows j0-1 .. j0+P, and the exchanges of row j0-1 with row j0
+#pragma unroll
+  for (int q = 0; q <= P + 1; q++) R[q] = load(j0 - 1 + q);
+  float pU, pUR, pUL; // exchanges of the previous row with the row above it
+  {
+    bool odd = false;
+    pU = th_pair_axial(R[0], R[1], md);
+    pUR = th_pair_diag<false>(R[0], th_right(R[1]), mdd, odd);
+    pUL = th_pair_diag<false>(R[0], th_left(R[1]), mdd, odd);
+    if (__builtin_expect(__any(odd), 0))
+    {
+      pUR = th_pair_diag<true>(R[0], th_right(R[1]), mdd, odd);
+      pUL = th_pair_diag<true>(R[0], th_left(R[1]), mdd, odd);
+    }
+  }
+  float f1 = 0.f, f2 = 0.f; // EXTRAP: final values of rows r-1 and r-2
+
+  for (int r0 = j0; r0 < j1; r0 += U)
+  {
+#pragma unroll
+    for (int q = 0; q < U; q++)
+    {
+      const int r = r0 + q;
+      if (r >= j1) break;
+      const float zm = R[q % U], z0 = R[(q + 1) % U], z1 = R[(q + 2) % U];
+      // the row P + 1 below the next one, into the slot of row r - 1
+      if (r + P + 1 <= j1) R[(q + P + 2) % U] = load(r + P + 1);
+
+      bool  odd = false;
+      float eR = th_pair_axial(z0, th_right(z0), md);
+      float eU = th_pair_axial(z0, z1, md);
+      float eUR = th_pair_diag<false>(z0, th_right(z1), mdd, odd);
+      float eUL = th_pair_diag<false>(z0, th_left(z1), mdd, odd);
+      if (__builtin_expect(__any(odd), 0))
+      {
+        eUR = th_pair_diag<true>(z0, th_right(z1), mdd, odd);
+        eUL = th_pair_diag<true>(z0, th_left(z1), mdd, odd);
+      }
+      // (-1, 0) as 0 - term: the empty asm keeps the compiler from folding
+      // (0 - a) + b into b - a, which differs from it in the sign of a zero
+      float amount = 0.f - th_left(eR); // (-1, 0)
+      asm volatile("" : "+v"(amount));
+      amount = amount + eU;             // (0, 1)
+      amount = amount - pU;             // (0, -1)
+      amount = amount + eR;             // (1, 0)
+      amount = amount - th_left(pUR);   // (-1, -1)
+      amount = amount + eUL;            // (-1, 1)
+      amount = amount - th_right(pUL);  // (1, -1)
+      amount = amount + eUR;            // (1, 1)
+      float step = z0 + amount;
+      // formed in every lane, before the store's EXEC mask (the lane
+      // exchanges above then fold into the additions)
+      asm volatile("" : "+v"(step));
+      pU = eU;
+      pUR = eUR;
+      pUL = eUL;
+
+      float *row = zn + (size_t)r * (size_t)nx;
+      if (!EXTRAP)
+      {
+        float out = step;
+        if (edge_strip || r == 0 || r == ny - 1)
+        {
+          // apply_boundaries (_common_index.cl:106-132): i == 0, i == nx - 1,
+          // j == 0, j == ny - 1 in that order copy the inward neighbour.
+          // The lane exchange runs in every lane before the selects (a DPP
+          // read from a lane the EXEC mask disables gives 0).
+          float zl = th_left(z0), zr = th_right(z0);
+          asm volatile("" : "+v"(zl), "+v"(zr));
+          if (r == 0) out = z1;
+          if (r == ny - 1) out = zm;
+          out = cn ? zl : out;
+          out = c0 ? zr : out;
+        }
+        if (writer) at_off(row, soff) = out;
+      }
+      else
+      {
+        // extrapolate_borders: columns (interior rows), then rows
+        float f = step;
+        if (edge_strip)
+        {
+          float s1r = th_right(step), s1l = th_left(step);
+          asm volatile("" : "+v"(s1r), "+v"(s1l));
+          float s2r = th_right(s1r), s2l = th_left(s1l);
+          asm volatile("" : "+v"(s2r), "+v"(s2l));
+          f = cn ? 2.f * s1l - s2l : f;
+          f = c0 ? 2.f * s1r - s2r : f;
+        }
+        if (r == ny - 1) f = 2.f * f1 - f2;
+        if (r == 2 && j0 == 0 && writer) at_off(zn, soff) = 2.f * f1 - f;
+        if (r != 0 && writer) at_off(row, soff) = f; // row 0 is written with row 2
+        f2 = f1;
+        f1 = f;
+      }
+    }
+  }
+}
+
+// Launch geometry of k_thermal_stream: strips of 62 columns and chunks sized
+// so that the work items just fill the device's wavefront slots (one round).
+template <int P, bool EXTRAP>
+static int launch_thermal_stream(const float *a, float *b, int nx, int ny, float ts, int num_cus,
+                                 hipStream_t st)
+{
+  static const int blocks_per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream<P, EXTRAP>, 256, 0) !=
+            hipSuccess ||
+        n < 1)
+    {
+      (void)hipGetLastError();
+      n = 2;
+    }
+    return n;
+  }();
+  const long strips = (nx + TH_OUT - 1) / TH_OUT;
+  // two rounds of wavefronts: 4096^2 takes 30.6-31.1 us per step with one,
+  // two or four rounds (two rows loaded ahead), 32.7 with eight
+  const long slots = 2 * 4L * blocks_per_cu * (num_cus > 0 ? num_cus : 256);
+  long       ch = (strips * ny + slots - 1) / slots;
+  ch = ch < 3 ? 3 : ch;
+  const long chunks = (ny + ch - 1) / ch;
+  const long items = strips * chunks;
+  const int  G = (int)((items + 3) / 4);
+  k_thermal_stream<P, EXTRAP><<<G, 256, 0, st>>>(a, b, nx, ny, ts, (int)strips, (int)ch,
+                                                 (int)items);
+  return check_launch("thermal_stream");
+}
+
+// Temporal-blocking depth used for uniform talus below 2^18 cells; 0
+// selects one step per launch (the streaming kernel) at every size.
 static int g_thermal_block_steps = 8;
 
-// Runs `iterations` Jacobi steps.  z holds the input; zscratch is a second
-// plane of the same size (see the declaration for where the result ends).
+// Runs `iterations` Jacobi steps, then (extrapolate) extrapolate_borders.
+// z holds the input; zscratch is a second plane of the same size (see the
+// declaration for where the result ends).
 int launch_thermal(hmg_ctx     *ctx,
                    float       *z,
                    float       *zscratch,
@@ -305,35 +564,28 @@ int launch_thermal(hmg_ctx     *ctx,
                    int          iterations,
                    const float *bedrock,
                    hipStream_t  st,
-                   float      **result)
+                   float      **result,
+                   bool         extrapolate)
 {
-  float *a = z, *b = zscratch;
-  int    done = 0;
+  float    *a = z, *b = zscratch;
+  int       done = 0;
+  bool      borders_done = false;
   const int T = g_thermal_block_steps;
-  // The blocked kernel saves launches and HBM passes but recomputes an
-  // 8-cell halo per tile; both kernels are VALU-bound, so it pays only where
-  // launches dominate.  50 steps, device-resident (ms): 256^2 blocked 3x
-  // faster; 512^2 0.196 blocked / 0.183 one step per launch; 1024^2 0.409 /
-  // 0.347; 4096^2 3.56 / 2.45.  Grids of 2^18 cells and more step one
-  // Jacobi step per launch.
+  const int cus = ctx && ctx->num_cus > 0 ? ctx->num_cus : 256;
+  // The blocked kernel recomputes an 8-cell halo per tile; it pays where
+  // launches dominate (below 2^18 cells).  From there the streaming kernel
+  // takes one step per launch.
+  const bool uniform = !talus && !bedrock && talus_scalar >= 0.f;
   const bool blocked_pays = (long)nx * ny < (1L << 18);
-  if (!talus && !bedrock && T == 8 && blocked_pays)
+  if (uniform && T == 8 && blocked_pays)
   {
-    // 64 x 16 tiles; grids too small to give every CU two of them (256^2
-    // gives 64) use 32 x 8 tiles, which recompute more halo per tile but
-    // occupy 4x as many CUs
+    // 32 x 8 tiles: at these sizes they give every CU a tile or more
     constexpr int TT = 8;
-    const int     cus = ctx && ctx->num_cus > 0 ? ctx->num_cus : 256;
-    const bool    small = (long)((nx + 63) / 64) * ((ny + 15) / 16) < 2L * cus;
     while (iterations - done >= 2)
     {
       int s = iterations - done < TT ? iterations - done : TT;
-      if (small)
-        k_thermal_blocked<32, 8, TT>
-            <<<dim3((nx + 31) / 32, (ny + 7) / 8), 256, 0, st>>>(a, b, nx, ny, talus_scalar, s);
-      else
-        k_thermal_blocked<64, 16, TT>
-            <<<dim3((nx + 63) / 64, (ny + 15) / 16), 256, 0, st>>>(a, b, nx, ny, talus_scalar, s);
+      k_thermal_blocked<32, 8, TT>
+          <<<dim3((nx + 31) / 32, (ny + 7) / 8), 256, 0, st>>>(a, b, nx, ny, talus_scalar, s);
       HMG_TRY(check_launch("thermal_blocked"));
       float *t = a;
       a = b;
@@ -341,16 +593,32 @@ int launch_thermal(hmg_ctx     *ctx,
       done += s;
     }
   }
-  dim3 blk(64, 4);
-  dim3 grd((nx + 63) / 64, (ny + 3) / 4);
+  const bool stream = uniform;
+  dim3       blk(64, 4);
+  dim3       grd((nx + 63) / 64, (ny + 3) / 4);
   for (; done < iterations; done++)
   {
-    k_thermal_step<<<grd, blk, 0, st>>>(a, b, nx, ny, talus, talus_scalar, bedrock);
-    HMG_TRY(check_launch("thermal_step"));
+    const bool last = done == iterations - 1;
+    // two rows loaded ahead: 4096^2 steps take 30.6 us, against 31.9 with
+    // four rows and 44-47 with eight or twelve (more registers, no faster
+    // memory)
+    if (stream && last && extrapolate && nx >= 4 && ny >= 4)
+    {
+      HMG_TRY((launch_thermal_stream<2, true>(a, b, nx, ny, talus_scalar, cus, st)));
+      borders_done = true;
+    }
+    else if (stream)
+      HMG_TRY((launch_thermal_stream<2, false>(a, b, nx, ny, talus_scalar, cus, st)));
+    else
+    {
+      k_thermal_step<<<grd, blk, 0, st>>>(a, b, nx, ny, talus, talus_scalar, bedrock);
+      HMG_TRY(check_launch("thermal_step"));
+    }
     float *t = a;
     a = b;
     b = t;
   }
+  if (extrapolate && !borders_done) HMG_TRY(launch_extrapolate_borders(a, nx, ny, st));
   if (result)
     *result = a;
   else if (a != z)
@@ -469,8 +737,7 @@ int hmg_thermal_dev(hmg_ctx     *ctx,
   DevBuf tmp;
   HMG_TRY(tmp.get(ctx->pool, (size_t)nx * ny * sizeof(float)));
   HMG_TRY(launch_thermal(ctx, z_dev, tmp.as<float>(), nx, ny, talus_dev, talus_scalar,
-                         iterations, bedrock_dev, ctx->stream));
-  HMG_TRY(launch_extrapolate_borders(z_dev, nx, ny, ctx->stream));
+                         iterations, bedrock_dev, ctx->stream, nullptr, true));
   // the scratch plane returns to the pool when this call ends
   HMG_HIP(hipStreamSynchronize(ctx->stream));
   return HMG_OK;
@@ -499,9 +766,8 @@ int hmg_thermal_dev_noalloc(hmg_ctx     *ctx,
   if (!zs) return set_error(HMG_ERR_DEVICE, "no scratch plane on device %d", ctx->device);
   // the Jacobi ping-pong runs through the device's static plane; an odd
   // number of launches copies the result back into z_dev (a device copy)
-  HMG_TRY(launch_thermal(ctx, z_dev, zs, nx, ny, talus_dev, talus_scalar, iterations,
-                         bedrock_dev, ctx->stream));
-  return launch_extrapolate_borders(z_dev, nx, ny, ctx->stream);
+  return launch_thermal(ctx, z_dev, zs, nx, ny, talus_dev, talus_scalar, iterations,
+                        bedrock_dev, ctx->stream, nullptr, true);
 }
 
 // Host-pointer form shared by thermal and thermal_auto_bedrock: upload,
@@ -559,15 +825,15 @@ static int thermal_host_call(hmg_ctx     *ctx,
     HMG_TRY(bed.get(ctx->pool, bytes));
     HMG_TRY(launch_thermal_auto(zb.as<float>(), tmp.as<float>(), bed.as<float>(), z0.as<float>(),
                                 nx, ny, pt, talus_scalar, iterations, st));
+    HMG_TRY(launch_extrapolate_borders(zb.as<float>(), nx, ny, st));
     HMG_HIP(hipStreamSynchronize(st)); // bed returns to the pool here
   }
   else
   {
     HMG_HIP(hipMemcpyAsync(z0.ptr, zb.ptr, bytes, hipMemcpyDeviceToDevice, st));
     HMG_TRY(launch_thermal(ctx, zb.as<float>(), tmp.as<float>(), nx, ny, pt, talus_scalar,
-                           iterations, pb, st));
+                           iterations, pb, st, nullptr, true));
   }
-  HMG_TRY(launch_extrapolate_borders(zb.as<float>(), nx, ny, st));
   if (pd || pm)
   {
     k_thermal_post<<<post_blocks(n), 256, 0, st>>>(zb.as<float>(), z0.as<float>(), pd, pm, n);
@@ -695,8 +961,7 @@ int hmg_sediment_deposition(hmg_ctx     *ctx,
     k_sd_pre<<<nb, 256, 0, st>>>(smap.as<float>(), zb.as<float>(), ztot.as<float>(), step, n);
     HMG_TRY(check_launch("sediment_deposition(pre)"));
     HMG_TRY(launch_thermal(ctx, ztot.as<float>(), tmp.as<float>(), nx, ny, pt, talus_scalar,
-                           thermal_subiterations, nullptr, st));
-    HMG_TRY(launch_extrapolate_borders(ztot.as<float>(), nx, ny, st));
+                           thermal_subiterations, nullptr, st, nullptr, true));
     k_sd_post<<<nb, 256, 0, st>>>(smap.as<float>(), ztot.as<float>(), zb.as<float>(), n);
     HMG_TRY(check_launch("sediment_deposition(post)"));
   }

@@ -1472,11 +1472,11 @@ int hmg_session_thermal(hmg_session *s, float talus, int iterations)
   float *zt = s->z[s->cur ^ 1].as<float>();
   float *res = zc;
   HMG_TRY(launch_thermal(s->ctx, zc, zt, s->nx, s->ny, nullptr, talus, iterations, nullptr,
-                         s->ctx->stream, &res));
+                         s->ctx->stream, &res, true));
   // the other z plane is free between iterations: after an odd number of
   // launches the result is there, and the two planes trade places
   if (res != zc) std::swap(s->z[s->cur].ptr, s->z[s->cur ^ 1].ptr);
-  return launch_extrapolate_borders(s->z[s->cur].as<float>(), s->nx, s->ny, s->ctx->stream);
+  return HMG_OK;
 }
 
 int hmg_session_export(hmg_session *s, float *z_host, float *d_host, float *s_host,

@@ -1,0 +1,90 @@
+"""The streaming Jacobi step (uniform talus, one launch per step, the
+thermal() call's extrapolate_borders folded into its last launch) against
+the FP32 oracle, bit for bit (compared as bit patterns, so the sign of a
+zero counts).  With block steps 0 every size takes the streaming kernel;
+the shapes cover one strip (nx <= 62), the strip edge (62, 63, 64, 124,
+125, 126), the moved last strip, grids of 3 to 6 rows, the last chunk
+shorter than 3 rows, zero talus, a talus no slope exceeds and signed
+zeros."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def streaming(hm):
+    L = hm.lib()
+    assert L.hmg_thermal_set_block_steps(0) == 0
+    yield
+    assert L.hmg_thermal_set_block_steps(8) == 0
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                          np.ascontiguousarray(b).view(np.uint32))
+
+
+def _input(orc, nx, ny, seed):
+    if nx >= 8 and ny >= 8:
+        return orc.fbm_terrain(nx, ny, seed)
+    return np.random.default_rng(seed).random((ny, nx), dtype=np.float32)
+
+
+SHAPES = [(3, 3), (4, 4), (5, 7), (7, 4), (62, 9), (63, 10), (64, 5), (65, 6), (124, 11),
+          (125, 12), (126, 13), (130, 67), (200, 131), (257, 3), (3, 257), (1000, 301)]
+
+
+@pytest.mark.parametrize("nx,ny", SHAPES)
+@pytest.mark.parametrize("iters", [1, 2, 5])
+def test_stream_steps_match_oracle(hm, orc, streaming, nx, ny, iters):
+    z0 = _input(orc, nx, ny, nx * 1000 + ny)
+    for talus in (0.5 / nx, 4.0 / nx):
+        z = z0.copy()
+        hm.thermal(z, talus, iters)
+        assert _bits_equal(z, orc.thermal(z0, talus, iters)), talus
+
+
+@pytest.mark.parametrize("talus", [0.0, 1e30])
+def test_stream_zero_and_huge_talus(hm, orc, streaming, talus):
+    z0 = _input(orc, 131, 77, 5)
+    z = z0.copy()
+    hm.thermal(z, talus, 3)
+    assert _bits_equal(z, orc.thermal(z0, talus, 3))
+
+
+def test_stream_signed_zeros(hm, orc, streaming):
+    """A field of -0.0 with a spike: where every exchange is zero the
+    reference adds +0 to -0 and gets +0."""
+    z0 = np.full((40, 70), -0.0, np.float32)
+    z0[20, 30] = 1.0
+    z0[5, 5:9] = 0.0
+    for it in (1, 2):
+        z = z0.copy()
+        hm.thermal(z, 0.01, it)
+        assert _bits_equal(z, orc.thermal(z0, 0.01, it))
+
+
+def test_stream_default_path_large_grid(hm, orc):
+    """Default kernel choice: a grid of 2^18 cells or more streams."""
+    z0 = orc.fbm_terrain(1030, 777, 8)
+    for it in (1, 4):
+        z = z0.copy()
+        hm.thermal(z, 1.0 / 1030, it)
+        assert _bits_equal(z, orc.thermal(z0, 1.0 / 1030, it))
+
+
+def test_stream_in_session_matches_oracle(hm, orc, streaming):
+    from conftest import VP_WORKLOAD
+
+    nx, ny = 190, 97
+    z0 = orc.fbm_terrain(nx, ny, 2)
+    s = hm.Session(z0, **VP_WORKLOAD)
+    o = orc.VpState(z0, orc.vp_params(**VP_WORKLOAD))
+    for k in (3, 10, 4):
+        s.vpipes(k).thermal(1.0 / nx, 2)
+        o.run(k).thermal_step(1.0 / nx, 2)
+    g = s.export()
+    s.close()
+    for a, b in zip(g, (o.z, o.d, o.s, o.u, o.v)):
+        assert _bits_equal(a, b)
