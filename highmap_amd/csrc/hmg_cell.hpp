@@ -108,6 +108,42 @@ __device__ __forceinline__ T &at_off(T *base, uint32_t idx)
   return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T));
 }
 
+// Loads and stores of streamed planes with the nontemporal hint: the
+// streaming kernels read and write each such element once per launch, and
+// the hint keeps them from displacing lines that are read again (the
+// sediment plane the gathers sample).
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *base, uint32_t idx)
+{
+  const char *p = reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T);
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+  if constexpr (sizeof(T) == 16)
+    return __builtin_bit_cast(T, __builtin_nontemporal_load(reinterpret_cast<const u4 *>(p)));
+  else if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_nontemporal_load(reinterpret_cast<const u2 *>(p)));
+  else if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p)));
+  else
+    return __builtin_bit_cast(T, __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(p)));
+}
+
+template <class T>
+__device__ __forceinline__ void st_stream(T *base, uint32_t idx, T v)
+{
+  char *p = reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T);
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+  if constexpr (sizeof(T) == 16)
+    __builtin_nontemporal_store(__builtin_bit_cast(u4, v), reinterpret_cast<u4 *>(p));
+  else if constexpr (sizeof(T) == 8)
+    __builtin_nontemporal_store(__builtin_bit_cast(u2, v), reinterpret_cast<u2 *>(p));
+  else if constexpr (sizeof(T) == 4)
+    __builtin_nontemporal_store(__builtin_bit_cast(uint32_t, v), reinterpret_cast<uint32_t *>(p));
+  else
+    __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, v), reinterpret_cast<uint16_t *>(p));
+}
+
 // ----------------------------------------------------- sin/cos and hashing
 
 // Project-written sin/cos: Cody-Waite reduction by pi/2 with a three-part

@@ -383,8 +383,10 @@ int hmg_server_submit(hmg_server         *sv,
 {
   clear_error();
   if (!sv || !d || !z_out_host) return set_error(HMG_ERR_INVALID, "null argument");
-  if (d->nx < 3 || d->ny < 3 || (size_t)d->nx * (size_t)d->ny > ((size_t)1 << 31))
-    return set_error(HMG_ERR_INVALID, "job grid must be at least 3x3 and below 2^31 cells");
+  // jobs advance in batched streaming launches, which address planes with
+  // 32-bit offsets: at most 2^28 cells
+  if (d->nx < 3 || d->ny < 3 || (size_t)d->nx * (size_t)d->ny > ((size_t)1 << 28))
+    return set_error(HMG_ERR_INVALID, "job grid must be at least 3x3 and at most 2^28 cells");
   if (d->iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
   if (d->kind != HMG_JOB_VPIPES_THERMAL && d->kind != HMG_JOB_THERMAL)
     return set_error(HMG_ERR_INVALID, "unknown job kind %d", d->kind);

@@ -49,6 +49,11 @@ namespace hmg
 
 constexpr int NSLOT = 16; // fixed-point accumulators per slot set
 
+// The streaming kernels address planes as a scalar base plus a 32-bit byte
+// offset; a 16-byte flux quadruple keeps that below 2^32 up to 2^28 cells.
+// Larger grids run the plain passes (64-bit indexing).
+constexpr size_t STREAM_MAX_CELLS = (size_t)1 << 28;
+
 struct VpDyn
 {
   int    has_prev; // 0 on the first iteration of a session
@@ -330,9 +335,9 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
                                             const VpDyn   &y)
 {
   RowA     a;
-  Flux4<S> f = at_off(F, row_off);
-  a.z = at_off(z, row_off);
-  a.dr = S::ld(at_off(D, row_off));
+  Flux4<S> f = ld_stream(F, row_off);
+  a.z = ld_stream(z, row_off);
+  a.dr = S::ld(ld_stream(D, row_off));
   a.fl = S::ld(f.l);
   a.fr = S::ld(f.r);
   a.ft = S::ld(f.t);
@@ -518,13 +523,13 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
         if (writer)
         {
           const uint32_t c = (uint32_t)(r * nx + gx);
-          at_off(Do, c) = S::st(d2);
+          st_stream(Do, c, S::st(d2));
           Flux4<S> o;
           o.l = S::st(c_0[0]);
           o.r = S::st(c_0[1]);
           o.t = S::st(c_0[2]);
           o.b = S::st(c_0[3]);
-          at_off(Fo, c) = o;
+          st_stream(Fo, c, o);
         }
         continue;
       }
@@ -579,15 +584,21 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
       if (writer)
       {
         const uint32_t c = (uint32_t)(r * nx + gx);
-        at_off(zo, c) = zn;
+        // z, D and F are read once by the next launch, and the sediment
+        // many times (its gathers): the streamed planes carry the
+        // nontemporal hint, the sediment does not, so that it stays in
+        // the caches for the gathers.  At 4096^2 FP32 this takes a launch
+        // from 226 to 212 us; the hint on the loads alone makes it 235, on
+        // every store (the sediment's too) 229.
+        st_stream(zo, c, zn);
         at_off(So, c) = S::st(sn);
-        at_off(Do, c) = S::st(d2);
+        st_stream(Do, c, S::st(d2));
         Flux4<S> o;
         o.l = S::st(c_0[0]);
         o.r = S::st(c_0[1]);
         o.t = S::st(c_0[2]);
         o.b = S::st(c_0[3]);
-        at_off(Fo, c) = o;
+        st_stream(Fo, c, o);
         if (k.maintain) fix_accumulate(acc, d2 * k.ce);
       }
     }
@@ -1251,6 +1262,7 @@ static int session_create_common(hmg_ctx                 *ctx,
   else
     hmg_vpipes_defaults(&prm);
   HMG_TRY(validate_vp_params(prm));
+  if ((size_t)nx * (size_t)ny > STREAM_MAX_CELLS) prm.mode = HMG_VP_PLAIN;
 
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   HMG_TRY(ctx_enter(ctx));
@@ -1846,6 +1858,8 @@ int batch_advance(hmg_ctx           *ctx,
     if ((ss[i]->k.kexp == 1.f) != ke1)
       return set_error(HMG_ERR_INVALID, "batched maps must share k_discharge_exp == 1 or != 1");
     if (iters[i] < 0) return set_error(HMG_ERR_INVALID, "negative iteration count");
+    if ((size_t)ss[i]->nx * ss[i]->ny > STREAM_MAX_CELLS)
+      return set_error(HMG_ERR_INVALID, "batched maps are limited to 2^28 cells");
     if ((long)ss[i]->iter + iters[i] > (1L << 30))
       return set_error(HMG_ERR_INVALID, "iteration count too large");
   }
@@ -1929,6 +1943,8 @@ int hmg_batch_add(hmg_batch   *b,
   clear_error();
   if (!b) return set_error(HMG_ERR_INVALID, "null batch");
   if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
+  if (nx > 0 && ny > 0 && (size_t)nx * (size_t)ny > STREAM_MAX_CELLS)
+    return set_error(HMG_ERR_INVALID, "batched maps are limited to 2^28 cells");
   hmg_session *s = nullptr;
   HMG_TRY(hmg_session_create(b->ctx, z_host, nx, ny, &b->p, rain_host, b->storage, &s));
   b->maps.push_back(s);
@@ -2131,8 +2147,8 @@ int hmg_flow_simulation_dev(hmg_ctx     *ctx,
   if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
   if (!z_dev || !depth_map_dev || !d_out_dev) return set_error(HMG_ERR_INVALID, "null array");
   if (nx < 1 || ny < 1) return set_error(HMG_ERR_INVALID, "shape must be positive");
-  if ((size_t)nx * (size_t)ny > ((size_t)1 << 31))
-    return set_error(HMG_ERR_INVALID, "grid larger than 2^31 cells");
+  if ((size_t)nx * (size_t)ny > STREAM_MAX_CELLS)
+    return set_error(HMG_ERR_INVALID, "flow_simulation is limited to 2^28 cells");
   if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
   if (!(dt > 0.f)) return set_error(HMG_ERR_INVALID, "dt must be > 0");
   hmg_vpipes_params prm;
@@ -2223,8 +2239,8 @@ int hmg_flow_simulation(hmg_ctx     *ctx,
   if (!ctx) return set_error(HMG_ERR_INVALID, "null context");
   if (!z_host || !depth_map_host || !d_out_host) return set_error(HMG_ERR_INVALID, "null array");
   if (nx < 1 || ny < 1) return set_error(HMG_ERR_INVALID, "shape must be positive");
-  if ((size_t)nx * (size_t)ny > ((size_t)1 << 31))
-    return set_error(HMG_ERR_INVALID, "grid larger than 2^31 cells");
+  if ((size_t)nx * (size_t)ny > STREAM_MAX_CELLS)
+    return set_error(HMG_ERR_INVALID, "flow_simulation is limited to 2^28 cells");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   HMG_TRY(ctx_enter(ctx));
   hipStream_t  st = ctx->stream;

@@ -328,7 +328,7 @@ int hmg_vpipes(hmg_ctx                 *ctx,
  * hydraulic_vpipes (no erosion, evaporation or top-up), then
  * water_depth_dry_out(d, dry_out_ratio) when dry_out_ratio != 0.  Writes the
  * water depth (nx*ny floats) to d_out_host; z and depth_map are not
- * changed. */
+ * changed.  At most 2^28 cells. */
 int hmg_flow_simulation(hmg_ctx     *ctx,
                         const float *z_host,
                         int          nx,
@@ -411,7 +411,10 @@ int hmg_snow_simulation_dev(hmg_ctx     *ctx,
  * in device memory between calls, so that hydraulic iterations and thermal
  * steps can be interleaved on one continuous state without host transfers.
  * The water, sediment and flux planes are stored as `storage`; all
- * arithmetic is FP32; the water-volume reduction is exact fixed point. */
+ * arithmetic is FP32; the water-volume reduction is exact fixed point.
+ * Grids of more than 2^28 cells run the plain passes whatever mode the
+ * parameters ask for (the fused kernel addresses its planes with 32-bit
+ * offsets); results do not depend on the mode. */
 int hmg_session_create(hmg_ctx                 *ctx,
                        const float             *z_host,
                        int                      nx,
@@ -483,8 +486,8 @@ size_t hmg_session_bytes(int nx, int ny, int storage);
  * the per-map semantics are hmg_vpipes'. */
 typedef struct hmg_batch hmg_batch;
 int hmg_batch_create(hmg_ctx *ctx, int storage, const hmg_vpipes_params *p, hmg_batch **out);
-/* Adds a map (host arrays; rain may be NULL) to run `iterations`
- * iterations; *map_id receives its index. */
+/* Adds a map (host arrays; rain may be NULL, at most 2^28 cells) to run
+ * `iterations` iterations; *map_id receives its index. */
 int hmg_batch_add(hmg_batch   *b,
                   const float *z_host,
                   int          nx,
@@ -550,8 +553,8 @@ int hmg_server_create(hmg_ctx                 *ctx,
                       int                      storage,
                       const hmg_vpipes_params *p,
                       hmg_server             **out);
-/* Queues a job; z_out_host (nx*ny floats) must stay valid until the job is
- * done.  Returns immediately. */
+/* Queues a job (at least 3 x 3, at most 2^28 cells); z_out_host (nx*ny
+ * floats) must stay valid until the job is done.  Returns immediately. */
 int hmg_server_submit(hmg_server         *sv,
                       const hmg_job_desc *d,
                       const float        *z_host,
