@@ -39,7 +39,11 @@ ALG_BYTES = {"f32": 8 + 6 * 4 * 2, "bf16": 8 + 6 * 2 * 2, "f16": 8 + 6 * 2 * 2}
 
 
 def cpu_threads():
-    return max(1, min(16, os.cpu_count() or 1))
+    """Every host CPU this process may run on."""
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        return max(1, os.cpu_count() or 1)
 
 
 def cpu_bench(workload, nx, ny, iters, reps=3, threads=None):
@@ -53,17 +57,30 @@ def cpu_bench(workload, nx, ny, iters, reps=3, threads=None):
 
 
 def cpu_model():
+    """Model name, vendor family/model/stepping, clock and CPU counts from
+    /proc/cpuinfo."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    info.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    parts = [info.get("model name", "unknown")]
+    fam = ", ".join("%s %s" % (k, info[k]) for k in ("vendor_id", "cpu family", "model",
+                                                      "stepping") if k in info)
+    if fam:
+        parts.append("(%s)" % fam)
+    if "cpu MHz" in info:
+        parts.append("%s MHz" % info["cpu MHz"])
+    parts.append("%d logical CPUs, %d in this process's affinity set"
+                 % (os.cpu_count() or 0, cpu_threads()))
+    return " ".join(parts)
 
 
-def cpu_baseline(n, iters=20, reps=3):
+def cpu_baseline(n, iters=20, reps=5):
     r = cpu_bench("vpipes_thermal", n, n, iters, reps)
     # one thread on a shorter sample of the same grid (no thermal step falls in it)
     r1 = cpu_bench("vpipes_thermal", n, n, 4, 1, 1)
@@ -73,8 +90,9 @@ def cpu_baseline(n, iters=20, reps=3):
         "cores": r["threads"],
         "kind": "port",
         "sample": "%dx%d fBm seed 2, %d vpipes iterations with a thermal step every 10, "
-                  "FP32 CPU restatement of the reference passes built -O3 -Ofast -ffast-math, "
-                  "OpenMP %d threads, median of %d runs after 1 warm-up (%.2f s each)"
+                  "FP32 CPU restatement of the reference passes built -O3 -Ofast -ffast-math "
+                  "(serial FP32 volume sum as the reference's), OpenMP %d threads, median "
+                  "of %d runs after 1 warm-up (%.2f s each)"
                   % (n, n, iters, r["threads"], reps, r["median_s"]),
         "value_1_thread": round(r1["gcell_iter_s"], 5),
         "sample_1_thread": "same grid, 4 vpipes iterations, 1 thread, 1 run after 1 warm-up "
@@ -128,6 +146,13 @@ def run_headline(args):
     n = args.n
     stream = torch.cuda.ExternalStream(ctx.stream)
     z = device_terrain(torch, hm, ctx, n, n, 2)
+    if not args.chunk_rows and args.mode == "fused":
+        # the online launch-shape probes (iterations 20-55 of a session) run
+        # on a throw-away session of the same input; the measured session
+        # then starts from the process-wide cache and probes nothing
+        pre = hm.Session(z, storage=STORAGE[args.storage], mode=args.mode, **WORKLOAD)
+        pre.vpipes(60)
+        pre.close()
     sess = hm.Session(z, storage=STORAGE[args.storage], mode=args.mode, **WORKLOAD)
     if args.chunk_rows:
         sess.set_tuning(args.chunk_rows, args.prefetch)
@@ -141,6 +166,7 @@ def run_headline(args):
         step()
     sess.sync()
     torch.cuda.synchronize()
+    probing_in_timed_steps = sess.tuning()[2]
 
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -205,7 +231,9 @@ def run_headline(args):
             "storage": args.storage,
             "mode": args.mode,
             "launch_shape": {"chunk_rows": chunk_rows, "prefetch": prefetch,
-                             "source": "set" if args.chunk_rows else "online tuning"},
+                             "source": "set" if args.chunk_rows else
+                             "online tuning, cached from a 60-iteration run of the same input",
+                             "probes_in_timed_steps": probing_in_timed_steps},
         },
         # how far (cells) the last state's sediment backtraces reach: the
         # gathers of this workload land hundreds of cells away
@@ -239,7 +267,55 @@ def run_headline(args):
                             "alg_frac_of_copy": round(achieved / copy_gbs, 4)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)
+    if not args.no_extras:
+        out["batch"] = batch_summary(args)
+        out["served"] = served_summary(args)
     return out
+
+
+def batch_summary(args):
+    """The 96-map batch (config of run_batch) in a few numbers."""
+    r = run_batch(args)
+    return {"workload": r["config"]["workload"],
+            "gcell_iter_s": r["value"],
+            "per_map_plain_gcell_iter_s": r["per_map_plain_gcell_iter_s"],
+            "per_map_fused_gcell_iter_s": r["per_map_fused_gcell_iter_s"],
+            "speedup_vs_per_map_plain": r["speedup_vs_per_map_plain"],
+            "launch_shape": r["config"]["tuning"], "storage": args.storage}
+
+
+def served_summary(args, requests=2000, sub_requests=500, sub_rate=40.0):
+    """The served Poisson stream (2000 requests at 200/s, BF16 state, 3 x
+    4096^2 active cells at most) against sequential plain calls, and a
+    sub-capacity load point (the first 500 requests of the same generator
+    at 40/s), all replayed in real time."""
+    from highmap_amd import Storage, VpipesParams, serve, workloads
+
+    import highmap_amd as hm
+
+    hm.init_gpu(0)
+    params = VpipesParams.make(**WORKLOAD)
+    cap = 3 * 4096 * 4096
+    serve.warm_up(Storage.BF16, params)
+    m = serve.measure(hm, Storage.BF16, params, workloads.serve_requests(n=requests), cap)
+    m.pop("_main")
+    low = serve.measure(hm, Storage.BF16, params,
+                        workloads.serve_requests(n=sub_requests, rate=sub_rate), cap,
+                        sequential=False)
+    low.pop("_main")
+    return {"workload": "Poisson stream, rate 200/s (seed 12345), %d requests: side "
+                        "log-uniform 256-4096, 50-500 iterations, 70%% vpipes+thermal / 30%% "
+                        "thermal-only, BF16 state" % requests,
+            "p50_s": m["p50_latency_s"], "p99_s": m["p99_latency_s"],
+            "gcell_iter_s": m["achieved_gcell_iter_s"],
+            "offered_gcell_iter_s": m["offered_gcell_iter_s"],
+            "max_active_cells": cap,
+            "sequential_plain": m["sequential_plain"],
+            "alone_equals_mix": m["alone_equals_mix"],
+            "sub_capacity": {"requests": sub_requests, "rate_per_s": sub_rate,
+                             "offered_gcell_iter_s": low["offered_gcell_iter_s"],
+                             "gcell_iter_s": low["achieved_gcell_iter_s"],
+                             "p50_s": low["p50_latency_s"], "p99_s": low["p99_latency_s"]}}
 
 
 # ---------------------------------------------------------------- batch
@@ -806,6 +882,8 @@ def main():
                     choices=["headline", "batch", "serve", "precision", "thermal256",
                              "vpipes1024", "flow", "snow", "tiles", "copy", "ceiling"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline: skip the batch and served-mix measurements")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

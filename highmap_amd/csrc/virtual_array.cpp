@@ -14,6 +14,7 @@
 #include <limits>
 #include <stdexcept>
 #include <thread>
+#include <tuple>
 
 namespace hmap
 {
@@ -118,24 +119,28 @@ VirtualArray::VirtualArray(glm::ivec2 shape, glm::ivec2 tile_shape, int halo,
 {
 }
 
+// clone and copy_from start from an empty RAM tile store of the copied
+// geometry, whatever store the source uses (as HighMap's do,
+// virtual_array.cpp:76-94); the data, if asked for, is copied tile by tile.
+static std::unique_ptr<TileStorage> empty_ram_store(glm::ivec2 shape, glm::ivec2 tile_shape)
+{
+  return make_storage(shape, tile_shape, StorageMode::VA_RAM);
+}
+
 std::unique_ptr<VirtualArray> VirtualArray::clone(const ComputeMode &cm, bool deep_copy)
 {
-  auto va = std::make_unique<VirtualArray>(shape, bbox, tile_shape, halo,
-                                           std::unique_ptr<TileStorage>());
-  // the same storage kind, empty (virtual_array.cpp:76-94)
-  va->storage = make_storage(shape, tile_shape, StorageMode::VA_RAM);
-  if (deep_copy) copy_data(*this, *va, cm);
-  return va;
+  auto copy = std::make_unique<VirtualArray>(shape, bbox, tile_shape, halo,
+                                             empty_ram_store(shape, tile_shape));
+  if (deep_copy) copy_data(*this, *copy, cm);
+  return copy;
 }
 
 void VirtualArray::copy_from(VirtualArray &src, const ComputeMode &cm, bool copy_src_data)
 {
   if (this == &src) return;
-  shape = src.shape;
-  bbox = src.bbox;
-  tile_shape = src.tile_shape;
-  halo = src.halo;
-  storage = make_storage(shape, tile_shape, StorageMode::VA_RAM);
+  storage = empty_ram_store(src.shape, src.tile_shape);
+  std::tie(shape, bbox, tile_shape, halo) =
+      std::make_tuple(src.shape, src.bbox, src.tile_shape, src.halo);
   if (copy_src_data) copy_data(src, *this, cm);
 }
 
@@ -486,55 +491,58 @@ static float smoothstep5(float x)
   return x * x * x * (x * (x * 6.f - 15.f) + 10.f);
 }
 
+// The overlap band between two neighbouring tiles along one axis.  Along
+// that axis the first `halo` cells of `hi` (the right or upper tile) are the
+// grid cells [n - 2 halo, n - halo) of `lo` (n its extent): band position k
+// of both becomes lerp(lo's value, hi's value, w_k), w_k = smoothstep5(k /
+// (halo - 1)), so the seam fades from lo's result into hi's.  The weights
+// are formed once per band; each cell pair is independent.
+static void blend_overlap_band(Array &lo, Array &hi, int halo, bool along_x)
+{
+  std::vector<float> w((size_t)std::max(halo, 0));
+  for (int k = 0; k < halo; k++) w[(size_t)k] = smoothstep5(float(k) / float(halo - 1));
+  const int first = (along_x ? lo.shape.x : lo.shape.y) - 2 * halo; // band start in lo
+  const int across = along_x ? lo.shape.y : lo.shape.x;
+  for (int k = 0; k < halo; k++)
+    for (int c = 0; c < across; c++)
+    {
+      float      &a = along_x ? lo(first + k, c) : lo(c, first + k);
+      float      &b = along_x ? hi(k, c) : hi(c, k);
+      const float v = std::lerp(a, b, w[(size_t)k]);
+      b = v;
+      a = v;
+    }
+}
+
 void VirtualArray::smooth_overlap_buffers()
 {
-  // virtual_array.cpp:337-401: along x then along y, each overlap band of
-  // the right (upper) tile becomes the blend from the left (lower) tile's
-  // copy of those cells to its own, with weight smoothstep5(p / (halo - 1))
-  // (std::lerp), and the left (lower) tile takes the blended values
-  if (storage->max_live_tiles() < 2)
+  // semantics of HighMap's VirtualArray::smooth_overlap_buffers
+  // (virtual_array.cpp:337-401): every pair of neighbours along x, then
+  // every pair along y, blended as blend_overlap_band describes
+  const size_t live = storage->max_live_tiles();
+  if (live < 2)
   {
-    std::fprintf(stderr, "VirtualArray: smooth_overlap_buffers requires at least 2 tiles in "
-                         "memory, skipping\n");
+    std::fprintf(stderr,
+                 "hmap::VirtualArray: overlap smoothing needs two tiles resident at once, "
+                 "the tile storage keeps %zu; the overlaps are left as they are\n",
+                 live);
     return;
   }
-  const int nx = ceil_div(shape.x, tile_shape.x), ny = ceil_div(shape.y, tile_shape.y);
-  for (int ty = 0; ty < ny; ++ty)
-    for (int tx = 0; tx < nx - 1; ++tx)
-    {
-      const TileRegion r0 = tile_region_from_tile_coords(tx, ty);
-      const TileRegion r1 = tile_region_from_tile_coords(tx + 1, ty);
-      Array           &t0 = storage->get_tile(r0);
-      Array           &t1 = storage->get_tile(r1);
-      for (int p = 0; p < halo; p++)
-        for (int q = 0; q < t0.shape.y; q++)
-        {
-          const float r = smoothstep5(float(p) / float(halo - 1));
-          const int   pb = t0.shape.x - 2 * halo + p;
-          t1(p, q) = std::lerp(t0(pb, q), t1(p, q), r);
-          t0(pb, q) = t1(p, q);
-        }
-      storage->release_tile(r0);
-      storage->release_tile(r1);
-    }
-  for (int ty = 0; ty < ny - 1; ++ty)
-    for (int tx = 0; tx < nx; ++tx)
-    {
-      const TileRegion r0 = tile_region_from_tile_coords(tx, ty);
-      const TileRegion r1 = tile_region_from_tile_coords(tx, ty + 1);
-      Array           &t0 = storage->get_tile(r0);
-      Array           &t1 = storage->get_tile(r1);
-      for (int p = 0; p < t0.shape.x; p++)
-        for (int q = 0; q < halo; q++)
-        {
-          const float r = smoothstep5(float(q) / float(halo - 1));
-          const int   qb = t0.shape.y - 2 * halo + q;
-          t1(p, q) = std::lerp(t0(p, qb), t1(p, q), r);
-          t0(p, qb) = t1(p, q);
-        }
-      storage->release_tile(r0);
-      storage->release_tile(r1);
-    }
+  const int tiles_x = ceil_div(shape.x, tile_shape.x), tiles_y = ceil_div(shape.y, tile_shape.y);
+  auto blend = [&](int ax, int ay, int bx, int by, bool along_x)
+  {
+    const TileRegion ra = tile_region_from_tile_coords(ax, ay);
+    const TileRegion rb = tile_region_from_tile_coords(bx, by);
+    Array           &a = storage->get_tile(ra);
+    Array           &b = storage->get_tile(rb);
+    blend_overlap_band(a, b, halo, along_x);
+    storage->release_tile(ra);
+    storage->release_tile(rb);
+  };
+  for (int ty = 0; ty < tiles_y; ++ty)
+    for (int tx = 0; tx + 1 < tiles_x; ++tx) blend(tx, ty, tx + 1, ty, true);
+  for (int ty = 0; ty + 1 < tiles_y; ++ty)
+    for (int tx = 0; tx < tiles_x; ++tx) blend(tx, ty, tx, ty + 1, false);
 }
 
 void copy_data(VirtualArray &src, VirtualArray &dst, const ComputeMode &cm)

@@ -158,24 +158,16 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def bench(args):
-    import highmap_amd as hm
-
-    hm.init_gpu(0)
-    storage = {"f32": Storage.F32, "bf16": Storage.BF16, "f16": Storage.F16}[args.storage]
-    params = VpipesParams.make(water_height=1e-3, evap_rate=0.01, dt=0.5, k_capacity=0.1,
-                               k_erode=0.05, k_depose=0.05)
-    reqs = workloads.serve_requests(n=args.requests)
+def measure(hm, storage, params, reqs, cap=0, sequential=True, check_ids=None):
+    """Replays `reqs` (arrival times in seconds from the start) against a
+    server in real time and returns throughput and latency (arrival to the
+    result on the host), and, with `sequential`, the same requests run one
+    after another with plain launches, each alone, with FIFO queueing
+    computed from their measured service times."""
     cell_iters = sum(r["side"] ** 2 * r["iterations"] for r in reqs)
     span = reqs[-1]["arrival_s"]
-    check_ids = sorted({0, 1, 2, len(reqs) // 2, len(reqs) - 1})
-
-    # warm-up (kernels loaded, pool primed)
-    sv = Server(storage=storage, params=params)
-    for r in reqs[:2]:
-        sv.submit(256, 256, JOB_VPIPES_THERMAL, 20, 1.0 / 256, seed=r["seed"])
-    sv.wait_all()
-    sv.close()
+    if check_ids is None:
+        check_ids = sorted({0, 1, 2, len(reqs) // 2, len(reqs) - 1})
 
     def served(cap):
         sv = Server(storage=storage, params=params, max_active_cells=cap)
@@ -220,52 +212,86 @@ def bench(args):
         return {"rate": cell_iters / (last_done - first_submit) / 1e9, "lat": lat, "wall": wall,
                 "stats": stats, "kept": kept}
 
-    cap = int(getattr(args, "max_active_cells", 0) or 0)
     main = served(cap)
-    other = served(0) if cap else None
-    kept = main["kept"]
-    lat, wall, stats, served_rate = main["lat"], main["wall"], main["stats"], main["rate"]
+    out = {"requests": len(reqs), "arrival_span_s": round(span, 3),
+           "offered_gcell_iter_s": round(cell_iters / span / 1e9, 2),
+           "achieved_gcell_iter_s": round(main["rate"], 3),
+           "p50_latency_s": round(_pct(main["lat"], 50), 4),
+           "p99_latency_s": round(_pct(main["lat"], 99), 4),
+           "wall_s": round(main["wall"], 3),
+           "server_busy_s": round(main["stats"]["busy_s"], 3),
+           "server_ticks": main["stats"]["ticks"],
+           "max_active_cells": cap}
+    if sequential:
+        service = []
+        same = {}
+        for r in reqs:
+            t0 = time.monotonic()
+            z = run_alone(hm, r, storage, params, "plain")
+            service.append(time.monotonic() - t0)
+            if r["id"] in main["kept"]:
+                k = main["kept"][r["id"]]
+                same[str(r["id"])] = bool(k is not None and np.array_equal(k, z))
+        finish = 0.0
+        seq_lat = []
+        for r, t in zip(reqs, service):
+            finish = max(finish, r["arrival_s"]) + t
+            seq_lat.append(finish - r["arrival_s"])
+        out["sequential_plain"] = {"gcell_iter_s": round(cell_iters / finish / 1e9, 3),
+                                   "p50_latency_s": round(_pct(seq_lat, 50), 4),
+                                   "p99_latency_s": round(_pct(seq_lat, 99), 4),
+                                   "total_service_s": round(sum(service), 3)}
+        out["alone_equals_mix"] = same
+    out["_main"] = main
+    return out
 
-    # the same requests as sequential plain calls, each alone
-    service = []
-    same = {}
-    for r in reqs:
-        t0 = time.monotonic()
-        z = run_alone(hm, r, storage, params, "plain")
-        service.append(time.monotonic() - t0)
-        if r["id"] in kept:
-            same[r["id"]] = bool(kept[r["id"]] is not None and np.array_equal(kept[r["id"]], z))
-    finish = 0.0
-    seq_lat = []
-    for r, s in zip(reqs, service):
-        finish = max(finish, r["arrival_s"]) + s
-        seq_lat.append(finish - r["arrival_s"])
-    seq_rate = cell_iters / finish / 1e9
+
+def warm_up(storage, params):
+    """Kernels loaded and the buffer pool primed before a timed replay."""
+    sv = Server(storage=storage, params=params)
+    for seed in (1, 2):
+        sv.submit(256, 256, JOB_VPIPES_THERMAL, 20, 1.0 / 256, seed=seed)
+    sv.wait_all()
+    sv.close()
+
+
+def bench(args):
+    import highmap_amd as hm
+
+    hm.init_gpu(0)
+    storage = {"f32": Storage.F32, "bf16": Storage.BF16, "f16": Storage.F16}[args.storage]
+    params = VpipesParams.make(water_height=1e-3, evap_rate=0.01, dt=0.5, k_capacity=0.1,
+                               k_erode=0.05, k_depose=0.05)
+    reqs = workloads.serve_requests(n=args.requests)
+    warm_up(storage, params)
+    cap = int(getattr(args, "max_active_cells", 0) or 0)
+    m = measure(hm, storage, params, reqs, cap)
+    main = m.pop("_main")
+    other = None
+    if cap:
+        o = measure(hm, storage, params, reqs, 0, sequential=False)
+        o.pop("_main")
+        other = {"gcell_iter_s": o["achieved_gcell_iter_s"], "p50_latency_s": o["p50_latency_s"],
+                 "p99_latency_s": o["p99_latency_s"]}
     return {
         "metric": "served mix: Gcell-iter/s and p50/p99 latency (continuous batching)",
-        "value": round(served_rate, 3),
+        "value": round(main["rate"], 3),
         "unit": "Gcell-iter/s",
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "f32 math, %s storage" % {1: "bf16", 2: "f16"}.get(storage, "f32"),
         "data": "synthetic: Poisson stream (rate 200/s, seed 12345), fBm terrain per request",
-        "config": {"requests": len(reqs), "arrival_span_s": round(span, 3),
-                   "offered_gcell_iter_s": round(cell_iters / span / 1e9, 2),
+        "config": {"requests": len(reqs), "arrival_span_s": m["arrival_span_s"],
+                   "offered_gcell_iter_s": m["offered_gcell_iter_s"],
                    "mix": "70% vpipes+thermal (1:10) / 30% thermal-only",
                    "side": "log-uniform 256-4096 (multiples of 64)", "iterations": "50-500"},
-        "p50_latency_s": round(_pct(lat, 50), 4),
-        "p99_latency_s": round(_pct(lat, 99), 4),
-        "wall_s": round(wall, 3),
-        "server_busy_s": round(stats["busy_s"], 3),
-        "server_ticks": stats["ticks"],
-        "sequential_plain": {"gcell_iter_s": round(seq_rate, 3),
-                             "p50_latency_s": round(_pct(seq_lat, 50), 4),
-                             "p99_latency_s": round(_pct(seq_lat, 99), 4),
-                             "total_service_s": round(sum(service), 3)},
-        "alone_equals_mix": same,
+        "p50_latency_s": m["p50_latency_s"],
+        "p99_latency_s": m["p99_latency_s"],
+        "wall_s": m["wall_s"],
+        "server_busy_s": m["server_busy_s"],
+        "server_ticks": m["server_ticks"],
+        "sequential_plain": m["sequential_plain"],
+        "alone_equals_mix": m["alone_equals_mix"],
         "max_active_cells": cap,
-        "uncapped": None if other is None else {
-            "gcell_iter_s": round(other["rate"], 3),
-            "p50_latency_s": round(_pct(other["lat"], 50), 4),
-            "p99_latency_s": round(_pct(other["lat"], 99), 4)},
+        "uncapped": other,
     }

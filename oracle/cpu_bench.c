@@ -14,6 +14,8 @@
  *             snow            snow_simulation as ex_snow_simulation (snow
  *                             0.1, uniform fall, no melting, talus 2.5/nx,
  *                             defaults, post-filter on)
+ * The water-volume sum of each iteration is a serial FP32 accumulate, as the
+ * reference's host loop does it (methods.cpp:428-431).
  * Input: remap(noise_fbm(PERLIN, kw 4, seed 2)) to [0,1] (not timed).
  * Output: one JSON line with the median wall time of `reps` timed runs after
  * one warm-up run.

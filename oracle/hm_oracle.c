@@ -727,7 +727,16 @@ static double orc_sum_f64(const double *a, int nx, int ny)
 
 static double orc_sum_f32fix(const float *a, int nx, int ny)
 {
+#ifdef ORC_BENCH_FLOAT_ACCUMULATE
+  /* the CPU-baseline build only: the reference's own volume sum, a serial
+   * FP32 std::accumulate over the water plane (methods.cpp:428-431) */
+  float acc = 0.f;
+  const size_t n = (size_t)nx * (size_t)ny;
+  for (size_t k = 0; k < n; k++) acc += a[k];
+  return (double)acc;
+#else
   return orc_fixsum_f32(a, (size_t)nx * (size_t)ny);
+#endif
 }
 
 #define REAL float
