@@ -796,8 +796,8 @@ def stream_sweep(torch, gib=4, modes=("read", "write", "copy"), unrolls=(1, 2, 4
 def measured_copy_gbs(torch):
     """Read + write bytes per second of the fastest HBM copy kernel variant
     (2 GiB buffers), and that variant."""
-    rows = stream_sweep(torch, gib=2, modes=("copy",), unrolls=(2, 4, 8),
-                        blocks_per_cu=(0, 8), reps=3)["copy"]
+    rows = stream_sweep(torch, gib=2, modes=("copy",), unrolls=(1, 2, 4),
+                        blocks_per_cu=(0,), reps=3)["copy"]
     return rows[0]
 
 
@@ -865,7 +865,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=6)
-    ap.add_argument("--storage", default="f32", choices=sorted(STORAGE))
+    ap.add_argument("--storage", default=None, choices=sorted(STORAGE),
+                    help="water/sediment/flux storage (default f32; bf16 for --workload serve)")
     ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
     ap.add_argument("--chunk-rows", type=int, default=0,
                     help="headline: fixed rows per wavefront (0: online tuning)")
@@ -885,6 +886,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="headline: skip the batch and served-mix measurements")
     args = ap.parse_args()
+    if args.storage is None:
+        args.storage = "bf16" if args.workload == "serve" else "f32"
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
