@@ -187,6 +187,135 @@ void remap(Array &array, float vmin, float vmax)
   for (float &x : array.vector) x = x * scale + offset;
 }
 
+// ------------------------------------------------------------- resampling
+
+// Catmull-Rom through p[0..3] at x in [0, 1) (between p[1] and p[2]) with
+// the reference's types (HighMap/include/highmap/interpolate/
+// interpolate2d.hpp:107-113): the two differences of samples taken in
+// float, every other operation in double, the result rounded to float.
+static float catmull_rom(const float p[4], float x)
+{
+  const double xd = x;
+  const double c3 = 3.0 * (double)(p[1] - p[2]) + (double)p[3] - (double)p[0];
+  const double c2 = 2.0 * p[0] - 5.0 * p[1] + 4.0 * p[2] - (double)p[3] + xd * c3;
+  const double c1 = (double)(p[2] - p[0]) + xd * c2;
+  return (float)(p[1] + 0.5 * xd * c1);
+}
+
+// f00 + (f10 - f00) u + (f01 - f00) v + (f11 - f10 - f01 + f00) u v, in
+// float, left to right (interpolate2d.hpp:94-105)
+static float bilinear(float f00, float f10, float f01, float f11, float u, float v)
+{
+  const float a10 = f10 - f00, a01 = f01 - f00, a11 = f11 - f10 - f01 + f00;
+  return f00 + a10 * u + a01 * v + a11 * u * v;
+}
+
+float Array::get_value_bilinear_at(int i, int j, float u, float v) const
+{
+  if (i == shape.x - 1) i = shape.x - 2, u = 1.f;
+  if (j == shape.y - 1) j = shape.y - 2, v = 1.f;
+  const Array &a = *this;
+  return bilinear(a(i, j), a(i + 1, j), a(i, j + 1), a(i + 1, j + 1), u, v);
+}
+
+float Array::get_value_bicubic_at(int i, int j, float u, float v) const
+{
+  // four Catmull-Rom curves along j (one per column i - 1 .. i + 2), then
+  // one along i through their values
+  float col[4];
+  for (int m = 0; m < 4; m++)
+  {
+    const int ic = std::clamp(i + m - 1, 0, shape.x - 1);
+    float     p[4];
+    for (int n = 0; n < 4; n++) p[n] = (*this)(ic, std::clamp(j + n - 1, 0, shape.y - 1));
+    col[m] = catmull_rom(p, v);
+  }
+  return catmull_rom(col, u);
+}
+
+// Source coordinates (cell centres at integers) of the n_t target points of
+// one axis: x_i = s + i ((1 - s) / n_t), s = 0.5 / n_t, in float with i
+// converted first; coordinate x_i / (1 / n_s) - 0.5.
+static std::vector<float> sample_axis(int n_t, int n_s)
+{
+  std::vector<float> c((size_t)std::max(n_t, 0));
+  const float        s = 0.5f * (1.f / (float)n_t);
+  const float        step = (1.f - s) / (float)n_t;
+  const float        cell = 1.f / (float)n_s;
+  for (int i = 0; i < n_t; i++) c[(size_t)i] = (n_t == 1 ? s : s + (float)i * step) / cell - 0.5f;
+  return c;
+}
+
+template <class Sample>
+static Array resample(const Array &src, glm::ivec2 new_shape, Sample &&sample)
+{
+  Array out(new_shape);
+  if (out.vector.empty()) return out;
+  if (src.vector.empty()) throw std::invalid_argument("hmap::Array: resampling an empty array");
+  const std::vector<float> cx = sample_axis(new_shape.x, src.shape.x);
+  const std::vector<float> cy = sample_axis(new_shape.y, src.shape.y);
+  for (int j = 0; j < new_shape.y; j++)
+    for (int i = 0; i < new_shape.x; i++) out(i, j) = sample(cx[(size_t)i], cy[(size_t)j]);
+  return out;
+}
+
+Array Array::resample_to_shape(glm::ivec2 new_shape) const
+{
+  return resample_to_shape_bilinear(new_shape);
+}
+
+Array Array::resample_to_shape_bilinear(glm::ivec2 new_shape) const
+{
+  const Array &a = *this;
+  return resample(a, new_shape,
+                  [&a](float xc, float yc)
+                  {
+                    // the lower corner clamped into the grid before the
+                    // offsets are taken (interpolate_array.cpp:122-185)
+                    const int i0 = std::clamp((int)xc, 0, a.shape.x - 1);
+                    const int j0 = std::clamp((int)yc, 0, a.shape.y - 1);
+                    const int i1 = std::min(i0 + 1, a.shape.x - 1);
+                    const int j1 = std::min(j0 + 1, a.shape.y - 1);
+                    return bilinear(a(i0, j0), a(i1, j0), a(i0, j1), a(i1, j1), xc - (float)i0,
+                                    yc - (float)j0);
+                  });
+}
+
+Array Array::resample_to_shape_bicubic(glm::ivec2 new_shape) const
+{
+  const Array &a = *this;
+  return resample(a, new_shape,
+                  [&a](float xc, float yc)
+                  {
+                    // offsets from the unclamped corner (truncated toward
+                    // zero); the 4x4 reads are clamped
+                    // (interpolate_array.cpp:31-103)
+                    const int i0 = (int)xc, j0 = (int)yc;
+                    return a.get_value_bicubic_at(i0, j0, xc - (float)i0, yc - (float)j0);
+                  });
+}
+
+static void same_shape(const Array &a, const Array &b)
+{
+  if (!(a.shape == b.shape)) throw std::invalid_argument("hmap::Array: shapes differ");
+}
+
+Array Array::operator+(const Array &other) const
+{
+  same_shape(*this, other);
+  Array out(shape);
+  for (size_t k = 0; k < vector.size(); k++) out.vector[k] = vector[k] + other.vector[k];
+  return out;
+}
+
+Array Array::operator-(const Array &other) const
+{
+  same_shape(*this, other);
+  Array out(shape);
+  for (size_t k = 0; k < vector.size(); k++) out.vector[k] = vector[k] - other.vector[k];
+  return out;
+}
+
 namespace gpu
 {
 

@@ -7,12 +7,17 @@
 // regions, bounding boxes and blended overlaps come out the same.
 #include "hmap_virtual_array.hpp"
 
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <future>
 #include <limits>
+#include <sstream>
 #include <stdexcept>
+#include <system_error>
 #include <thread>
 #include <tuple>
 
@@ -85,12 +90,265 @@ std::unique_ptr<TileStorage> RamTileStorage::clone() const
   return std::make_unique<RamTileStorage>(*this);
 }
 
-std::unique_ptr<TileStorage> make_storage(glm::ivec2, glm::ivec2, StorageMode mode)
+// ------------------------------------------------------------ LRU storage
+
+LruTileStorage::LruTileStorage(size_t max_tiles) : max_tiles(std::max<size_t>(1, max_tiles)) {}
+
+Array &LruTileStorage::get_tile(const TileRegion &region)
 {
-  if (mode != StorageMode::VA_RAM)
-    throw std::invalid_argument("hmap::VirtualArray: only the RAM tile storage is provided");
-  return std::make_unique<RamTileStorage>();
+  std::lock_guard<std::mutex> lk(mutex);
+  auto                        it = tiles.find(region.key);
+  if (it == tiles.end())
+  {
+    // make room first: least recently used tiles that nobody holds leave
+    // until the new one fits (or only held tiles are left)
+    while (tiles.size() >= max_tiles)
+    {
+      auto victim = tiles.end();
+      for (auto e = tiles.begin(); e != tiles.end(); ++e)
+        if (e->second.in_use == 0 &&
+            (victim == tiles.end() || e->second.last_use < victim->second.last_use))
+          victim = e;
+      if (victim == tiles.end()) break;
+      on_evict(victim->first, victim->second.tile);
+      tiles.erase(victim);
+    }
+    Entry e;
+    e.tile = Array(region.shape);
+    on_create(region, e.tile);
+    it = tiles.emplace(region.key, std::move(e)).first;
+  }
+  it->second.in_use++;
+  it->second.last_use = ++clock;
+  return it->second.tile;
 }
+
+void LruTileStorage::release_tile(const TileRegion &region)
+{
+  std::lock_guard<std::mutex> lk(mutex);
+  auto                        it = tiles.find(region.key);
+  if (it != tiles.end() && it->second.in_use > 0) it->second.in_use--;
+}
+
+size_t LruTileStorage::max_live_tiles() const
+{
+  return max_tiles;
+}
+
+size_t LruTileStorage::resident_tiles() const
+{
+  std::lock_guard<std::mutex> lk(mutex);
+  return tiles.size();
+}
+
+std::unique_ptr<TileStorage> LruTileStorage::clone() const
+{
+  return std::make_unique<LruTileStorage>(max_tiles);
+}
+
+std::string LruTileStorage::info_string() const
+{
+  return "LRU (capacity=" + std::to_string(max_tiles) + ")";
+}
+
+void LruTileStorage::on_evict(const TileKey &, const Array &) {}
+
+bool LruTileStorage::on_create(const TileRegion &, Array &)
+{
+  return false;
+}
+
+// ------------------------------------------------------------ disk storages
+
+// A new directory under the system's temporary directory, named from the
+// prefix, the process id and a per-process counter (retried past names
+// that exist).
+static std::filesystem::path fresh_temp_dir(const char *prefix)
+{
+  static std::atomic<unsigned> counter{0};
+  const auto                   base = std::filesystem::temp_directory_path();
+  for (;;)
+  {
+    const auto dir = base / (std::string(prefix) + "_" + std::to_string((long)::getpid()) + "_" +
+                             std::to_string(counter.fetch_add(1)));
+    if (std::filesystem::create_directory(dir)) return dir;
+  }
+}
+
+static std::filesystem::path tile_file(const std::filesystem::path &root, const TileKey &key)
+{
+  return root / ("tile_" + std::to_string(key.tx) + "_" + std::to_string(key.ty) + ".f32");
+}
+
+static void write_tile(const std::filesystem::path &path, const Array &tile)
+{
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("hmap::VirtualArray: cannot write " + path.string());
+  const size_t n = std::fwrite(tile.vector.data(), sizeof(float), tile.vector.size(), f);
+  const bool   ok = std::fclose(f) == 0 && n == tile.vector.size();
+  if (!ok) throw std::runtime_error("hmap::VirtualArray: short write to " + path.string());
+}
+
+// Reads the tile if its file exists (it must hold exactly the tile's cells).
+static bool read_tile(const std::filesystem::path &path, Array &tile)
+{
+  FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  const size_t n = std::fread(tile.vector.data(), sizeof(float), tile.vector.size(), f);
+  const bool   whole = n == tile.vector.size() && std::fgetc(f) == EOF;
+  std::fclose(f);
+  if (!whole) throw std::runtime_error("hmap::VirtualArray: " + path.string() +
+                                       " does not hold the tile's cells");
+  return true;
+}
+
+DiskLruTileStorage::DiskLruTileStorage(size_t max_tiles)
+    : LruTileStorage(max_tiles), root(fresh_temp_dir("hmap_va_lru"))
+{
+}
+
+DiskLruTileStorage::~DiskLruTileStorage()
+{
+  std::error_code ec;
+  std::filesystem::remove_all(root, ec);
+}
+
+std::unique_ptr<TileStorage> DiskLruTileStorage::clone() const
+{
+  return std::make_unique<DiskLruTileStorage>(max_tiles);
+}
+
+std::string DiskLruTileStorage::info_string() const
+{
+  return "DiskLRU (capacity=" + std::to_string(max_tiles) + ")";
+}
+
+void DiskLruTileStorage::on_evict(const TileKey &key, const Array &tile)
+{
+  write_tile(tile_file(root, key), tile);
+}
+
+bool DiskLruTileStorage::on_create(const TileRegion &region, Array &tile)
+{
+  return read_tile(tile_file(root, region.key), tile);
+}
+
+void DiskLruTileStorage::trim()
+{
+  std::lock_guard<std::mutex> lk(mutex);
+  for (auto it = tiles.begin(); it != tiles.end();)
+    if (it->second.in_use == 0)
+    {
+      on_evict(it->first, it->second.tile);
+      it = tiles.erase(it);
+    }
+    else
+      ++it;
+}
+
+DiskSequentialTileStorage::DiskSequentialTileStorage() : root(fresh_temp_dir("hmap_va_seq")) {}
+
+DiskSequentialTileStorage::~DiskSequentialTileStorage()
+{
+  std::error_code ec;
+  std::filesystem::remove_all(root, ec);
+}
+
+std::unique_ptr<TileStorage> DiskSequentialTileStorage::clone() const
+{
+  // an empty store with a directory of its own
+  return std::make_unique<DiskSequentialTileStorage>();
+}
+
+Array &DiskSequentialTileStorage::get_tile(const TileRegion &region)
+{
+  std::lock_guard<std::mutex> lk(mutex);
+  if (live)
+    throw std::logic_error("hmap::VirtualArray: the disk-sequential tile storage holds one tile; "
+                           "tile " + std::to_string(live_key.tx) + "," +
+                           std::to_string(live_key.ty) + " is not released yet");
+  Array tile(region.shape);
+  read_tile(tile_file(root, region.key), tile);
+  live_key = region.key;
+  live = std::move(tile);
+  return *live;
+}
+
+void DiskSequentialTileStorage::write_live()
+{
+  write_tile(tile_file(root, live_key), *live);
+  live.reset();
+}
+
+void DiskSequentialTileStorage::release_tile(const TileRegion &region)
+{
+  std::lock_guard<std::mutex> lk(mutex);
+  if (!live)
+  {
+    std::fprintf(stderr, "hmap::VirtualArray: disk-sequential release of tile %d,%d, none held\n",
+                 region.key.tx, region.key.ty);
+    return;
+  }
+  if (!(region.key == live_key))
+    std::fprintf(stderr,
+                 "hmap::VirtualArray: disk-sequential release of tile %d,%d while %d,%d is held; "
+                 "the held tile is written back\n",
+                 region.key.tx, region.key.ty, live_key.tx, live_key.ty);
+  write_live();
+}
+
+void DiskSequentialTileStorage::trim()
+{
+  std::lock_guard<std::mutex> lk(mutex);
+  if (live) write_live();
+}
+
+std::unique_ptr<TileStorage> make_storage(glm::ivec2 shape, glm::ivec2 tile_shape,
+                                          StorageMode mode)
+{
+  // tile_storage.cpp:12-38
+  switch (mode)
+  {
+  case StorageMode::VA_RAM: return std::make_unique<RamTileStorage>();
+  case StorageMode::VA_DISK_LRU:
+    return std::make_unique<DiskLruTileStorage>(
+        (size_t)ceil_div(shape.x, tile_shape.x) * (size_t)ceil_div(shape.y, tile_shape.y));
+  case StorageMode::VA_DISK_LRU_MIN: return std::make_unique<DiskLruTileStorage>(2);
+  case StorageMode::VA_DISK_SEQUENTIAL: return std::make_unique<DiskSequentialTileStorage>();
+  }
+  throw std::invalid_argument("hmap::VirtualArray: unknown storage mode " +
+                              std::to_string((int)mode));
+}
+
+std::string to_string(StorageMode m)
+{
+  switch (m)
+  {
+  case StorageMode::VA_RAM: return "VA_RAM";
+  case StorageMode::VA_DISK_LRU: return "VA_DISK_LRU";
+  case StorageMode::VA_DISK_LRU_MIN: return "VA_DISK_LRU_MIN";
+  case StorageMode::VA_DISK_SEQUENTIAL: return "VA_DISK_SEQUENTIAL";
+  }
+  return "UNKNOWN";
+}
+
+std::string to_string(ForEachMode m)
+{
+  switch (m)
+  {
+  case ForEachMode::VA_SEQUENTIAL: return "VA_SEQUENTIAL";
+  case ForEachMode::VA_DISTRIBUTED: return "VA_DISTRIBUTED";
+  case ForEachMode::VA_SINGLE_ARRAY: return "VA_SINGLE_ARRAY";
+  case ForEachMode::VA_SINGLE_ARRAY_DOWNSCALED: return "VA_SINGLE_ARRAY_DOWNSCALED";
+  }
+  return "UNKNOWN";
+}
+
+const std::map<std::string, int> for_each_mode_as_string = {
+    {"Distributed", ForEachMode::VA_DISTRIBUTED},
+    {"Sequential", ForEachMode::VA_SEQUENTIAL},
+    {"Single array", ForEachMode::VA_SINGLE_ARRAY},
+    {"Single array downscaled", ForEachMode::VA_SINGLE_ARRAY_DOWNSCALED}};
 
 // ----------------------------------------------------------- VirtualArray
 
@@ -267,6 +525,69 @@ void VirtualArray::from_array(const Array &array, const ComputeMode &cm)
       }
   };
   for_each_tile(*this, lambda, cm);
+}
+
+// Cell centre (global index + 0.5) scaled to the array: source position
+// x = (i + 0.5) (n_a - 1) / (n - 1), split into the floor and the offset
+// (virtual_array.cpp:136-226).
+template <class Sample>
+static void from_array_sampled(VirtualArray &va, const Array &array, const ComputeMode &cm,
+                               Sample &&sample)
+{
+  auto lambda = [&](Array &tile, const TileRegion &region)
+  {
+    const float      rx = float(array.shape.x - 1) / float(va.shape.x - 1);
+    const float      ry = float(array.shape.y - 1) / float(va.shape.y - 1);
+    const glm::ivec2 ij0 = va.tile_region_global_indices(region);
+    for (int j = 0; j < region.shape.y; ++j)
+      for (int i = 0; i < region.shape.x; ++i)
+      {
+        const float xs = (float(ij0.x + i) + 0.5f) * rx, ys = (float(ij0.y + j) + 0.5f) * ry;
+        const int   ig = int(std::floor(xs)), jg = int(std::floor(ys));
+        tile(i, j) = sample(ig, jg, xs - float(ig), ys - float(jg));
+      }
+  };
+  for_each_tile(va, lambda, cm);
+}
+
+void VirtualArray::from_array_bilinear(const Array &array, const ComputeMode &cm)
+{
+  // the corner clamped so that the 2x2 cells exist; offsets kept
+  from_array_sampled(*this, array, cm,
+                     [&array](int ig, int jg, float u, float v)
+                     {
+                       return array.get_value_bilinear_at(std::clamp(ig, 0, array.shape.x - 2),
+                                                          std::clamp(jg, 0, array.shape.y - 2),
+                                                          u, v);
+                     });
+}
+
+void VirtualArray::from_array_bicubic(const Array &array, const ComputeMode &cm)
+{
+  // the corner clamped into the array; offsets kept
+  from_array_sampled(*this, array, cm,
+                     [&array](int ig, int jg, float u, float v)
+                     {
+                       return array.get_value_bicubic_at(std::clamp(ig, 0, array.shape.x - 1),
+                                                         std::clamp(jg, 0, array.shape.y - 1),
+                                                         u, v);
+                     });
+}
+
+std::string VirtualArray::info_string(const ComputeMode &cm, int indent) const
+{
+  // the fields of virtual_array.cpp:285-309
+  const std::string  pad((size_t)std::max(indent, 0), ' ');
+  const glm::ivec2   nt = get_max_tiles();
+  std::ostringstream o;
+  o << pad << "VirtualArray @" << static_cast<const void *>(this) << "\n"
+    << pad << "  shape        : " << shape.x << " x " << shape.y << "\n"
+    << pad << "  tile_size    : " << tile_shape.x << " x " << tile_shape.y << "\n"
+    << pad << "  tile_grid    : " << nt.x << " x " << nt.y << " (" << nt.x * nt.y << " tiles)\n"
+    << pad << "  halo         : " << halo << "\n"
+    << pad << "  storage      : " << (storage ? storage->info_string() : "<none>") << "\n"
+    << pad << "  value_range  : [" << min(cm) << ", " << max(cm) << "]\n";
+  return o.str();
 }
 
 Array VirtualArray::to_array(const glm::ivec2 &array_shape, const ComputeMode &cm) const
@@ -567,6 +888,61 @@ void remap(Array &array, float vmin, float vmax, float from_min, float from_max)
 
 // ------------------------------------------------------------- tile loops
 
+// The single-array modes (virtual_array.inl:225-375): gather every array,
+// resample it bilinearly to the working shape, run `func` once on the whole
+// arrays, and put the outputs back as the mode says.
+static void single_array_loop(const TileAccess &access, const TileFunc &func,
+                              const ComputeMode &cm)
+{
+  const VirtualArray *ref = access.ref_va();
+  const bool          down = cm.mode == ForEachMode::VA_SINGLE_ARRAY_DOWNSCALED;
+  const ComputeMode   local{ForEachMode::VA_DISTRIBUTED, false, 1, 1.f};
+  glm::ivec2          wrk = ref->shape;
+  if (!down && cm.stride > 1)
+    wrk = glm::ivec2(std::max(2, int(ref->shape.x / float(cm.stride))),
+                     std::max(2, int(ref->shape.y / float(cm.stride))));
+  if (down && cm.k_cutoff != 1.f)
+    wrk = glm::ivec2(std::max(2, int(ref->shape.x * cm.k_cutoff)),
+                     std::max(2, int(ref->shape.y * cm.k_cutoff)));
+
+  const size_t       ni = access.inputs.size(), no = access.outputs.size();
+  std::vector<Array> ain(ni), aout(no);
+  std::vector<const Array *> in(ni, nullptr);
+  std::vector<Array *>       out(no, nullptr);
+  for (size_t k = 0; k < ni; ++k)
+    if (access.inputs[k])
+    {
+      ain[k] = access.inputs[k]->to_array(local).resample_to_shape(wrk);
+      in[k] = &ain[k];
+    }
+  for (size_t k = 0; k < no; ++k)
+    if (access.outputs[k])
+    {
+      aout[k] = access.outputs[k]->to_array(local).resample_to_shape(wrk);
+      out[k] = &aout[k];
+    }
+
+  func(in, out, TileRegion(TileKey{0, 0}, ref->bbox, wrk, glm::ivec4(0, 0, 0, 0)));
+
+  for (size_t k = 0; k < no; ++k)
+  {
+    VirtualArray *va = access.outputs[k];
+    if (!va) continue;
+    if (down)
+    {
+      // coarse result at full size, plus the full array's detail above the
+      // working grid's band (its own down-then-up round trip removed)
+      const Array full = va->to_array(local);
+      const Array low = full.resample_to_shape(wrk).resample_to_shape_bicubic(ref->shape);
+      va->from_array(aout[k].resample_to_shape_bicubic(ref->shape) + (full - low), local);
+    }
+    else if (wrk == ref->shape)
+      va->from_array(aout[k], local);
+    else
+      va->from_array_bicubic(aout[k], local);
+  }
+}
+
 void for_each_tile(const TileAccess &access, const TileFunc &func, const ComputeMode &cm)
 {
   // virtual_array.inl:19-109
@@ -587,15 +963,35 @@ void for_each_tile(const TileAccess &access, const TileFunc &func, const Compute
 
   auto dispatch = [&](const TileRegion &region)
   {
-    std::vector<const Array *> in;
-    std::vector<Array *>       out;
-    for (auto *va : access.inputs) in.push_back(va ? &va->storage->get_tile(region) : nullptr);
-    for (auto *va : access.outputs) out.push_back(va ? &va->storage->get_tile(region) : nullptr);
-    func(in, out, region);
-    for (auto *va : access.inputs)
-      if (va) va->storage->release_tile(region);
-    for (auto *va : access.outputs)
-      if (va) va->storage->release_tile(region);
+    // every tile taken is given back, also when a storage or func throws
+    std::vector<const VirtualArray *> held;
+    auto                              give_back = [&]()
+    {
+      for (const VirtualArray *va : held) va->storage->release_tile(region);
+      held.clear();
+    };
+    try
+    {
+      std::vector<const Array *> in;
+      std::vector<Array *>       out;
+      for (auto *va : access.inputs)
+      {
+        in.push_back(va ? &va->storage->get_tile(region) : nullptr);
+        if (va) held.push_back(va);
+      }
+      for (auto *va : access.outputs)
+      {
+        out.push_back(va ? &va->storage->get_tile(region) : nullptr);
+        if (va) held.push_back(va);
+      }
+      func(in, out, region);
+    }
+    catch (...)
+    {
+      give_back();
+      throw;
+    }
+    give_back();
   };
 
   const int ntx = ceil_div(ref->shape.x, ref->tile_shape.x);
@@ -624,27 +1020,11 @@ void for_each_tile(const TileAccess &access, const TileFunc &func, const Compute
     break;
   }
   case ForEachMode::VA_SINGLE_ARRAY:
-  {
-    if (cm.stride > 1)
-      throw std::invalid_argument("for_each_tile: strided VA_SINGLE_ARRAY is not provided");
-    const ComputeMode  local{ForEachMode::VA_DISTRIBUTED, false, 1, 1.f};
-    std::vector<Array> ain, aout;
-    for (auto *va : access.inputs) ain.push_back(va ? va->to_array(local) : Array());
-    for (auto *va : access.outputs) aout.push_back(va ? va->to_array(local) : Array());
-    std::vector<const Array *> in;
-    std::vector<Array *>       out;
-    for (size_t k = 0; k < access.inputs.size(); ++k)
-      in.push_back(access.inputs[k] ? &ain[k] : nullptr);
-    for (size_t k = 0; k < access.outputs.size(); ++k)
-      out.push_back(access.outputs[k] ? &aout[k] : nullptr);
-    const TileRegion whole(TileKey{0, 0}, ref->bbox, ref->shape, glm::ivec4(0, 0, 0, 0));
-    func(in, out, whole);
-    for (size_t k = 0; k < access.outputs.size(); ++k)
-      if (access.outputs[k]) access.outputs[k]->from_array(aout[k], local);
+  case ForEachMode::VA_SINGLE_ARRAY_DOWNSCALED:
+    single_array_loop(access, func, cm);
     break;
-  }
   default:
-    throw std::invalid_argument("for_each_tile: VA_SINGLE_ARRAY_DOWNSCALED is not provided");
+    throw std::invalid_argument("for_each_tile: unknown mode " + std::to_string((int)cm.mode));
   }
   if (cm.trim_storage)
   {

@@ -109,6 +109,28 @@ public:
   // a dtype other than little-endian float32.
   void to_numpy(const std::string &fname) const;
   void from_numpy(const std::string &fname);
+
+  // Sampling between cells, (i, j) the lower corner, (u, v) the offsets
+  // (HighMap/src/array/methods.cpp:203-245): bilinear over the 2x2 cells
+  // (a corner on the last row or column steps one cell back with offset 1),
+  // bicubic Catmull-Rom over the 4x4 cells around, indices clamped.
+  float get_value_bilinear_at(int i, int j, float u, float v) const;
+  float get_value_bicubic_at(int i, int j, float u, float v) const;
+
+  // Resampling to another shape (HighMap/src/array/methods.cpp:364-383 over
+  // hmap::interpolate_array_bilinear / _bicubic with their defaults,
+  // HighMap/src/interpolate/interpolate_array.cpp:15-185): target point i
+  // of n_t sits at 0.5/n_t + i (1 - 0.5/n_t) / n_t of the unit square (the
+  // reference's grid: pixel-centred start, end point excluded), i.e. at
+  // source index x n_s - 0.5.  resample_to_shape is the bilinear form.  The
+  // same shape is not an identity: the last cell samples half a cell in.
+  Array resample_to_shape(glm::ivec2 new_shape) const;
+  Array resample_to_shape_bilinear(glm::ivec2 new_shape) const;
+  Array resample_to_shape_bicubic(glm::ivec2 new_shape) const;
+
+  // Cell by cell; the shapes must agree (std::invalid_argument otherwise).
+  Array operator+(const Array &other) const;
+  Array operator-(const Array &other) const;
 };
 
 // hmap::remap (HighMap/src/range/range.cpp:342-361).

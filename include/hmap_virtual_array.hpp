@@ -10,20 +10,27 @@
 //   VirtualArray               HighMap/include/highmap/virtual_array/virtual_array.hpp:27-165
 //   for_each_tile, for_each_cell HighMap/include/highmap/virtual_array/virtual_array.inl
 //
-// Provided: RAM tile storage; the VA_SEQUENTIAL, VA_DISTRIBUTED and
-// VA_SINGLE_ARRAY (stride 1) compute modes.  Not provided (they throw
-// std::invalid_argument): the disk storages and the strided and downscaled
-// single-array modes (they need HighMap's resampling).
+// Provided: every tile storage (RAM, RAM LRU, disk-backed LRU with the
+// capacity of the whole grid or of two tiles, disk-sequential) and every
+// compute mode (sequential, distributed, single array at full size or
+// strided, single array downscaled).
 //
 // Tiles of one for_each_tile call in VA_DISTRIBUTED mode run on separate
 // host threads; the hmap::gpu functions give each calling thread its own
 // device context (stream and buffer pool), so those tiles' uploads, kernels
-// and downloads overlap on the GPU.
+// and downloads overlap on the GPU.  The LRU storages never evict a tile
+// that is between get_tile and release_tile, so they are safe in that mode
+// too (the store holds more tiles than its capacity while every resident
+// tile is in use).
 #pragma once
 
+#include <cstdint>
+#include <filesystem>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <optional>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -72,10 +79,12 @@ struct TileRegion
 enum StorageMode : int
 {
   VA_RAM,
-  VA_DISK_LRU,
-  VA_DISK_LRU_MIN,
-  VA_DISK_SEQUENTIAL
+  VA_DISK_LRU,       // disk-backed LRU, capacity the number of tiles
+  VA_DISK_LRU_MIN,   // disk-backed LRU, capacity two tiles
+  VA_DISK_SEQUENTIAL // one tile in memory; no overlap smoothing
 };
+
+std::string to_string(StorageMode m);
 
 class TileStorage
 {
@@ -86,11 +95,13 @@ public:
   virtual size_t                       max_live_tiles() const = 0;
   virtual std::unique_ptr<TileStorage> clone() const = 0;
   virtual std::string                  info_string() const = 0;
-  virtual void                         trim() {}
+  // drop what memory can be dropped, keeping the data
+  virtual void trim() {}
 };
 
 // Every tile in host memory, created (zero-filled, halo included) on first
 // access.  Safe for concurrent access to distinct tiles.
+// (HighMap/src/virtual_array/ram_tile_storage.cpp)
 class RamTileStorage : public TileStorage
 {
 public:
@@ -108,6 +119,98 @@ private:
   std::mutex                                      mutex;
 };
 
+// At most `max_tiles` tiles in memory (more only while every resident tile
+// is in use); the least recently used tile not in use leaves when another
+// is needed.  This base class drops the tile that leaves: a later access
+// finds it zero-filled again (HighMap/src/virtual_array/
+// lru_tile_storage.cpp).  Thread-safe.
+class LruTileStorage : public TileStorage
+{
+public:
+  explicit LruTileStorage(size_t max_tiles);
+
+  Array                       &get_tile(const TileRegion &region) override;
+  void                         release_tile(const TileRegion &region) override;
+  size_t                       max_live_tiles() const override;
+  std::unique_ptr<TileStorage> clone() const override;
+  std::string                  info_string() const override;
+
+  size_t resident_tiles() const; // tiles in memory now
+
+protected:
+  struct Entry
+  {
+    Array         tile;
+    int           in_use = 0;   // get_tile calls not yet released
+    std::uint64_t last_use = 0; // access clock
+  };
+
+  // With the lock held: a tile leaves memory / a new tile is created (return
+  // false to leave it zero-filled).
+  virtual void on_evict(const TileKey &key, const Array &tile);
+  virtual bool on_create(const TileRegion &region, Array &tile);
+
+  size_t                                          max_tiles;
+  std::unordered_map<TileKey, Entry, TileKeyHash> tiles;
+  std::uint64_t                                   clock = 0;
+  mutable std::mutex                              mutex;
+};
+
+// The LRU store with the tiles that leave memory written to a private
+// directory under the system's temporary directory (one raw float32 file
+// per tile) and read back on the next access; trim() writes every resident
+// tile out and frees it.  The directory is removed with the store.
+// (HighMap/src/virtual_array/disk_lru_storage.cpp)
+class DiskLruTileStorage : public LruTileStorage
+{
+public:
+  explicit DiskLruTileStorage(size_t max_tiles);
+  ~DiskLruTileStorage() override;
+
+  std::unique_ptr<TileStorage> clone() const override;
+  std::string                  info_string() const override;
+  void                         trim() override;
+
+  const std::filesystem::path &directory() const { return root; }
+
+protected:
+  void on_evict(const TileKey &key, const Array &tile) override;
+  bool on_create(const TileRegion &region, Array &tile) override;
+
+private:
+  std::filesystem::path root;
+};
+
+// One tile in memory at a time: get_tile reads it from disk (zero-filled
+// the first time), release_tile writes it back and frees it.  A second
+// get_tile before the release throws std::logic_error, so only sequential
+// loops (and single-cell get/set) may use it; overlap smoothing, which needs
+// two tiles at once, is skipped with a message.
+// (HighMap/src/virtual_array/disk_sequential_storage.cpp)
+class DiskSequentialTileStorage : public TileStorage
+{
+public:
+  DiskSequentialTileStorage();
+  ~DiskSequentialTileStorage() override;
+
+  Array                       &get_tile(const TileRegion &region) override;
+  void                         release_tile(const TileRegion &region) override;
+  size_t                       max_live_tiles() const override { return 1; }
+  std::unique_ptr<TileStorage> clone() const override;
+  std::string                  info_string() const override { return "DiskSequential"; }
+  void                         trim() override;
+
+  const std::filesystem::path &directory() const { return root; }
+
+private:
+  std::filesystem::path root;
+  std::mutex            mutex;
+  TileKey               live_key{0, 0};
+  std::optional<Array>  live;
+
+  void write_live();
+};
+
 std::unique_ptr<TileStorage> make_storage(glm::ivec2 shape, glm::ivec2 tile_shape,
                                           StorageMode mode);
 
@@ -115,16 +218,22 @@ enum ForEachMode : int
 {
   VA_SEQUENTIAL,             // tile by tile, one thread
   VA_DISTRIBUTED,            // tile by tile, hardware_concurrency threads
-  VA_SINGLE_ARRAY,           // the whole array at once
-  VA_SINGLE_ARRAY_DOWNSCALED // not provided
+  VA_SINGLE_ARRAY,           // the whole array at once (every `stride`-th cell)
+  VA_SINGLE_ARRAY_DOWNSCALED // the whole array at once, downscaled by k_cutoff
 };
+
+std::string to_string(ForEachMode m);
+
+// names of the modes as a user interface lists them
+// (HighMap/include/highmap/virtual_array/virtual_array.hpp:50-54)
+extern const std::map<std::string, int> for_each_mode_as_string;
 
 struct ComputeMode
 {
   ForEachMode mode;
   bool        trim_storage = false;
-  int         stride = 1;
-  float       k_cutoff = 1.f;
+  int         stride = 1;     // VA_SINGLE_ARRAY: shape / stride cells (at least 2)
+  float       k_cutoff = 1.f; // VA_SINGLE_ARRAY_DOWNSCALED: shape * k_cutoff cells
 };
 
 struct VirtualArray
@@ -153,8 +262,14 @@ struct VirtualArray
   glm::ivec2 get_max_tiles() const;
   int        get_ntiles() const;
 
+  std::string info_string(const ComputeMode &cm, int indent = 0) const;
+
   void  fill(float value, const ComputeMode &cm);
+  // Every cell (halo included) from the array scaled to this grid: the
+  // nearest cell, or the bilinear / bicubic sample at the cell centre.
   void  from_array(const Array &array, const ComputeMode &cm);
+  void  from_array_bilinear(const Array &array, const ComputeMode &cm);
+  void  from_array_bicubic(const Array &array, const ComputeMode &cm);
   Array to_array(const glm::ivec2 &array_shape, const ComputeMode &cm) const;
   Array to_array(const ComputeMode &cm) const;
   Array to_array_dbg() const;
@@ -209,9 +324,20 @@ using TileFunc = std::function<void(const std::vector<const Array *> &,
                                     std::vector<Array *> &,
                                     const TileRegion &)>;
 
-// The general loop: `func` receives the input and output tiles of one
-// region (halo included).  Throws std::invalid_argument for arrays of other
-// shapes or tile shapes and for the modes not provided.
+// The general loop (HighMap/include/highmap/virtual_array/virtual_array.inl).
+// Tile modes: `func` receives the input and output tiles of one region
+// (halo included).  Single-array modes: `func` receives whole arrays of a
+// working shape and a region covering the grid's bbox without halo:
+//   VA_SINGLE_ARRAY             working shape max(2, shape / stride) per
+//       axis (the shape itself at stride 1); every array is gathered and
+//       resampled bilinearly to it (at stride 1 too, as the reference
+//       does, which shifts cells by up to half a cell); the outputs come
+//       back cell for cell, or bicubically when strided;
+//   VA_SINGLE_ARRAY_DOWNSCALED  working shape max(2, shape * k_cutoff) per
+//       axis (the shape at k_cutoff 1); the outputs come back as the
+//       working result upsampled bicubically plus the detail the working
+//       grid cannot hold: full - bicubic_up(bilinear_down(full)).
+// Throws std::invalid_argument for arrays of other shapes or tile shapes.
 void for_each_tile(const TileAccess &access, const TileFunc &func, const ComputeMode &cm);
 
 template <typename Func>

@@ -4,9 +4,12 @@ operation by operation, for checking the drop-in's C++ VirtualArray.
 
 Reference: HighMap/src/virtual_array/virtual_array.cpp,
 virtual_array_processing.cpp, HighMap/include/highmap/virtual_array/
-virtual_array.inl, tile_region.hpp; std::lerp as libstdc++ defines it
-(<cmath>, C++20).  Arrays are numpy (ny, nx) float32 planes, so element
-(i, j) of hmap::Array is a[j, i].
+virtual_array.inl, tile_region.hpp; the resampling of
+HighMap/src/array/methods.cpp:203-245,364-383 and
+HighMap/src/interpolate/interpolate_array.cpp:15-185 with
+HighMap/include/highmap/interpolate/interpolate2d.hpp:94-113; std::lerp as
+libstdc++ defines it (<cmath>, C++20).  Arrays are numpy (ny, nx) float32
+planes, so element (i, j) of hmap::Array is a[j, i].
 """
 import numpy as np
 
@@ -85,6 +88,43 @@ class VA:
         ig = np.clip(np.floor(xs * rx).astype(np.int64), 0, nx_a - 1)
         jg = np.clip(np.floor(ys * ry).astype(np.int64), 0, ny_a - 1)
         return ii, jj, ig, jg
+
+    def from_array_bicubic(self, a):
+        """virtual_array.cpp:176-213: the bicubic sample at each cell centre
+        scaled to the array, x = (i + 0.5) (n_a - 1) / (n - 1), corner
+        floor(x) clamped into the array, offset x - floor(x) kept."""
+        ny_a, nx_a = a.shape
+        rx = F(F(nx_a - 1) / F(self.shape[0] - 1))
+        ry = F(F(ny_a - 1) / F(self.shape[1] - 1))
+        for r in self.regions():
+            i0, j0 = self.origin(r)
+            xs = ((i0 + np.arange(r.shape[0])).astype(np.float32) + F(0.5)) * rx
+            ys = ((j0 + np.arange(r.shape[1])).astype(np.float32) + F(0.5)) * ry
+            ig, jg = np.floor(xs).astype(np.int64), np.floor(ys).astype(np.int64)
+            u, v = (xs - ig.astype(np.float32)), (ys - jg.astype(np.float32))
+            self.tile(r)[...] = bicubic_at(a, np.clip(ig, 0, nx_a - 1)[None, :],
+                                           np.clip(jg, 0, ny_a - 1)[:, None],
+                                           u[None, :], v[:, None])
+
+    def single_array(self, func, stride=1, k_cutoff=None):
+        """virtual_array.inl:225-375 for one output array: VA_SINGLE_ARRAY
+        (k_cutoff None) or VA_SINGLE_ARRAY_DOWNSCALED.  func takes and
+        returns a (ny, nx) float32 array of the working shape."""
+        nx, ny = self.shape
+        wrk = (nx, ny)
+        if k_cutoff is None and stride > 1:
+            wrk = (max(2, int(F(nx) / F(stride))), max(2, int(F(ny) / F(stride))))
+        if k_cutoff is not None and F(k_cutoff) != F(1):
+            wrk = (max(2, int(F(nx) * F(k_cutoff))), max(2, int(F(ny) * F(k_cutoff))))
+        full = self.to_array()
+        out = np.asarray(func(resample_bilinear(full, wrk)), np.float32)
+        if k_cutoff is not None:
+            low = resample_bicubic(resample_bilinear(full, wrk), (nx, ny))
+            self.from_array(resample_bicubic(out, (nx, ny)) + (full - low))
+        elif wrk == (nx, ny):
+            self.from_array(out)
+        else:
+            self.from_array_bicubic(out)
 
     def from_array(self, a):
         """virtual_array.cpp:104-130 (nearest source cell of each centre,
@@ -233,6 +273,77 @@ class VA:
                 qb = t0.shape[0] - 2 * hw
                 t1[:hw, :] = lerp(t0[qb:qb + hw, :], t1[:hw, :], ramp[:, None])
                 t0[qb:qb + hw, :] = t1[:hw, :]
+
+
+def catmull_rom(p0, p1, p2, p3, x):
+    """interpolate2d.hpp:107-113, elementwise: the differences p1 - p2 and
+    p2 - p0 in float32, every other operation in float64 in the source's
+    order, the result rounded to float32."""
+    p = [np.asarray(q, np.float32) for q in (p0, p1, p2, p3)]
+    d12 = (p[1] - p[2]).astype(np.float64)
+    d20 = (p[2] - p[0]).astype(np.float64)
+    P0, P1, P2, P3 = (q.astype(np.float64) for q in p)
+    x = np.asarray(x, np.float32).astype(np.float64)
+    c3 = 3.0 * d12 + P3 - P0
+    c2 = 2.0 * P0 - 5.0 * P1 + 4.0 * P2 - P3 + x * c3
+    c1 = d20 + x * c2
+    return (P1 + 0.5 * x * c1).astype(np.float32)
+
+
+def bilinear(f00, f10, f01, f11, u, v):
+    """interpolate2d.hpp:94-105 in float32, left to right."""
+    a10, a01 = f10 - f00, f01 - f00
+    a11 = ((f11 - f10) - f01) + f00
+    return (((f00 + a10 * u) + a01 * v) + (a11 * u) * v).astype(np.float32)
+
+
+def bicubic_at(a, i, j, u, v):
+    """Array::get_value_bicubic_at (methods.cpp:203-223) for broadcastable
+    index and offset arrays: Catmull-Rom along j in the columns i-1..i+2,
+    then along i; reads clamped into the array."""
+    ny, nx = a.shape
+    col = []
+    for m in range(-1, 3):
+        ic = np.clip(i + m, 0, nx - 1)
+        p = [a[np.clip(j + n, 0, ny - 1), ic] for n in range(-1, 3)]
+        col.append(catmull_rom(*p, v))
+    return catmull_rom(*col, u)
+
+
+def sample_axis(nt, ns):
+    """Source coordinates of the nt target points of one axis
+    (interpolate_array.cpp with endpoint false, pixel-centred, unit boxes):
+    s = 0.5 / nt, x_i = s + i ((1 - s) / nt), coordinate x_i / (1 / ns) - 0.5,
+    all float32."""
+    s = F(F(0.5) * F(F(1) / F(nt)))
+    step = F(F(F(1) - s) / F(nt))
+    x = s + np.arange(nt).astype(np.float32) * step if nt > 1 else np.array([s], np.float32)
+    return (x.astype(np.float32) / F(F(1) / F(ns))).astype(np.float32) - F(0.5)
+
+
+def resample_bilinear(a, shape):
+    """Array::resample_to_shape (interpolate_array.cpp:122-185): the lower
+    corner truncated and clamped before the offsets are taken."""
+    ny, nx = a.shape
+    cx, cy = sample_axis(shape[0], nx), sample_axis(shape[1], ny)
+    i0 = np.clip(cx.astype(np.int64), 0, nx - 1)
+    j0 = np.clip(cy.astype(np.int64), 0, ny - 1)
+    u = (cx - i0.astype(np.float32))[None, :]
+    v = (cy - j0.astype(np.float32))[:, None]
+    i1, j1 = np.minimum(i0 + 1, nx - 1), np.minimum(j0 + 1, ny - 1)
+    g = lambda jj, ii: a[jj[:, None], ii[None, :]]
+    return bilinear(g(j0, i0), g(j0, i1), g(j1, i0), g(j1, i1), u, v)
+
+
+def resample_bicubic(a, shape):
+    """Array::resample_to_shape_bicubic (interpolate_array.cpp:31-103): the
+    corner truncated toward zero, not clamped, for the offsets."""
+    ny, nx = a.shape
+    cx, cy = sample_axis(shape[0], nx), sample_axis(shape[1], ny)
+    i0, j0 = cx.astype(np.int64), cy.astype(np.int64)
+    u = (cx - i0.astype(np.float32))[None, :]
+    v = (cy - j0.astype(np.float32))[:, None]
+    return bicubic_at(a, i0[None, :], j0[:, None], u, v)
 
 
 def _last(idx):

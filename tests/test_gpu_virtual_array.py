@@ -1,6 +1,9 @@
 """hmap::gpu functions run tile by tile over a hmap::VirtualArray (the C++
 drop-in), as examples/ex_virtual_array does, in sequential and distributed
-modes: every tile bit for bit equal to the FP32 oracle run on that tile."""
+modes, with RAM tiles and with the disk-backed store that keeps two tiles in
+memory: every tile bit for bit equal to the FP32 oracle run on that tile.
+Thermal in the downscaled single-array mode (half the cells per axis)
+equals the oracle's thermal on the restated resampling."""
 import os
 import subprocess
 
@@ -51,6 +54,15 @@ def want(orc):
 
 
 @pytest.mark.parametrize("stage", ["noise", "thermal", "vpipes", "final"])
-@pytest.mark.parametrize("mode", ["seq", "dist"])
+@pytest.mark.parametrize("mode", ["seq", "dist", "disk"])
 def test_tiles_match_oracle(out, want, stage, mode):
     assert np.array_equal(out(stage + "_" + mode), want[stage]), (stage, mode)
+
+
+def test_thermal_in_downscaled_mode_matches_oracle(out, orc):
+    va = V.VA((1024, 1024), (0.0, 1.0, 0.0, 1.0), (220, 312), 32)
+    for r in va.regions():
+        va.tile(r)[...] = orc.noise_fbm(r.shape[0], r.shape[1], (2.0, 2.0), 0, 8, 0.7, 0.5, 2.0,
+                                        bbox=r.bbox)
+    va.single_array(lambda a: orc.thermal(a, F(2.0) / F(512.0), 20), k_cutoff=0.5)
+    assert np.array_equal(out("thermal_downscaled"), va.to_array())

@@ -2,8 +2,13 @@
 against the restatement in oracle/virtual_array.py, bit for bit, and pins
 of the restatement: tiles partition the grid, halos only towards
 neighbours, bounding boxes span their cells, same-shape round trips are the
-identity, std::lerp as the C++ library computes it, and overlap smoothing
-keeps agreeing overlaps unchanged.  No GPU call."""
+identity, std::lerp as the C++ library computes it, overlap smoothing
+keeps agreeing overlaps unchanged, resampling reproduces constant, linear
+and (bicubic) quadratic fields at the reference's sample positions, and
+the downscaled single-array mode returns the array unchanged when the
+operation is the identity.  The tile storages (RAM, disk-backed LRU,
+disk-sequential) give the same grid as RAM, and their eviction, reload,
+trim and one-tile rules hold.  No GPU call."""
 import os
 import subprocess
 
@@ -202,8 +207,209 @@ def test_cpp_virtual_array_matches_restatement(run, k):
     assert np.array_equal(load("smooth_tiles_%d" % k), tiles)
     vs.remap(0.0, 1.0)
     assert np.array_equal(load("smooth_remap_%d" % k, (ny, nx)), vs.to_array())
-    # the single-array mode: the whole array at once, then back to tiles
-    ii = np.arange(nx)[None, :]
-    jj = np.arange(ny)[:, None]
-    want = src + (ii * ny + jj).astype(np.float32)
-    assert np.array_equal(load("single_%d" % k, (ny, nx)), want)
+
+
+# ------------------------------------------------ resampling and the modes
+
+
+@pytest.fixture(scope="module")
+def modes(tmp_path_factory):
+    d = tmp_path_factory.mktemp("vam")
+    r = subprocess.run([DRIVER, "modes", str(d)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    def load(name, shape):
+        return np.fromfile(os.path.join(d, name + ".f32"), dtype=np.float32).reshape(
+            shape[1], shape[0])
+
+    return load
+
+
+def _exact_axis(nt, ns):
+    """The reference grid's source coordinates in float64."""
+    s = 0.5 / nt
+    return (s + np.arange(nt) * (1 - s) / nt) * ns - 0.5
+
+
+@pytest.mark.parametrize("nt,ns", [(7, 20), (40, 13), (64, 64), (1, 5), (333, 77)])
+def test_sample_positions(nt, ns):
+    """Pixel-centred start 0.5/nt, end point excluded: the same shape does
+    not sample the cell centres (the last cell sits at n - 1.5 + 0.5/n)."""
+    c = V.sample_axis(nt, ns)
+    assert np.allclose(c, _exact_axis(nt, ns), atol=2e-5 * ns)
+    if nt == ns:
+        assert c[0] == 0 and abs(c[-1] - (ns - 1.5 + 0.5 / ns)) < 1e-4
+
+
+def test_resampling_reproduces_polynomials():
+    """Constants exactly; a linear field (different slopes along i and j)
+    bilinearly at every point inside the source; a quadratic field
+    bicubically where the 4x4 stencil stays inside (Catmull-Rom has
+    quadratic precision)."""
+    ns = (37, 23)
+    i = np.arange(ns[0], dtype=np.float64)[None, :]
+    j = np.arange(ns[1], dtype=np.float64)[:, None]
+    c = np.full((ns[1], ns[0]), 0.3, np.float32)
+    for shape in ((50, 31), (12, 9), (37, 23)):
+        assert np.all(V.resample_bilinear(c, shape) == F(0.3))
+        assert np.all(V.resample_bicubic(c, shape) == F(0.3))
+        cx, cy = _exact_axis(shape[0], ns[0]), _exact_axis(shape[1], ns[1])
+        lin = (0.25 * i + 1.5 * j).astype(np.float32)
+        got = V.resample_bilinear(lin, shape)
+        inx = (cx >= 0) & (cx <= ns[0] - 1)
+        iny = (cy >= 0) & (cy <= ns[1] - 1)
+        want = 0.25 * cx[None, :] + 1.5 * cy[:, None]
+        assert np.allclose(got[np.ix_(iny, inx)], want[np.ix_(iny, inx)], atol=2e-4)
+        quad = (0.02 * i * i + 0.5 * j).astype(np.float32)
+        got = V.resample_bicubic(quad, shape)
+        okx = (cx >= 1) & (np.floor(cx) <= ns[0] - 3)
+        oky = (cy >= 1) & (np.floor(cy) <= ns[1] - 3)
+        want = 0.02 * cx[None, :] ** 2 + 0.5 * cy[:, None]
+        assert np.allclose(got[np.ix_(oky, okx)], want[np.ix_(oky, okx)], atol=2e-4)
+
+
+def test_bicubic_interpolates_at_the_nodes():
+    a = pattern(19, 11)
+    i = np.arange(19)[None, :]
+    j = np.arange(11)[:, None]
+    z = np.zeros((1, 1), np.float32)
+    assert np.array_equal(V.bicubic_at(a, i, j, z, z), a)
+    # and a cubic in x is not reproduced (the pin tells the orders apart)
+    x = np.float32(0.25)
+    assert V.catmull_rom(-1.0, 0.0, 1.0, 8.0, x) == F(0.109375)  # x - 3x^2 + 3x^3, not x^3
+    assert V.catmull_rom(1.0, 0.0, 1.0, 4.0, x) == F(0.0625)     # x^2 exactly
+
+
+def test_downscaled_identity_keeps_the_array():
+    """With an operation that changes nothing, coarse + (full - low) is the
+    full array up to the rounding of one subtraction and one addition."""
+    va = V.VA((130, 97), (0, 1, 0, 1), (40, 33), 5)
+    a = pattern(130, 97)
+    va.from_array(a)
+    va.single_array(lambda x: x, k_cutoff=0.5)
+    assert np.allclose(va.to_array(), a, atol=4e-7)
+
+
+@pytest.mark.parametrize("k", [0, 2, 3, 4])  # 1 is 0's grid in 8 x 8 tiles
+def test_cpp_resampling_and_modes_match_restatement(modes, k):
+    shape, bbox, tile, halo = GEOS[k]
+    nx, ny = shape
+    src = pattern(nx, ny)
+    down = (nx // 3 + 1, ny // 2 + 5)
+    up = (2 * nx - 3, ny + 7)
+    assert np.array_equal(modes("rs_lin_down_%d" % k, down), V.resample_bilinear(src, down))
+    assert np.array_equal(modes("rs_lin_up_%d" % k, up), V.resample_bilinear(src, up))
+    assert np.array_equal(modes("rs_lin_same_%d" % k, shape), V.resample_bilinear(src, shape))
+    assert np.array_equal(modes("rs_cub_down_%d" % k, down), V.resample_bicubic(src, down))
+    assert np.array_equal(modes("rs_cub_up_%d" % k, up), V.resample_bicubic(src, up))
+    small = pattern(nx // 3 + 2, ny // 2 + 3)
+    va = V.VA(shape, bbox, tile, halo)
+    va.from_array_bicubic(small)
+    assert np.array_equal(modes("from_cub_%d" % k, shape), va.to_array())
+
+    def add_index(a):
+        h, w = a.shape
+        return a + (np.arange(w)[None, :] * h + np.arange(h)[:, None]).astype(np.float32)
+
+    for name, kw, f in (("single", {}, add_index), ("single_s3", {"stride": 3}, add_index),
+                        ("down_05", {"k_cutoff": 0.5}, lambda a: a * F(0.5)),
+                        ("down_1", {"k_cutoff": 1.0}, lambda a: a * F(0.5))):
+        vc = V.VA(shape, bbox, tile, halo)
+        vc.from_array(src)
+        vc.single_array(f, **kw)
+        assert np.array_equal(modes("%s_%d" % (name, k), shape), vc.to_array()), name
+
+
+def test_cpp_from_array_bilinear(modes):
+    """from_array_bilinear: corner clamped to n - 2, so the 2x2 cells exist;
+    on a linear source it lands on the scaled cell centre."""
+    nx, ny = 130, 97
+    va_lin = modes("from_lin_2", (nx, ny))
+    sx, sy = 130 // 3 + 2, 97 // 2 + 3
+    assert np.all(np.isfinite(va_lin))
+    # the pattern is piecewise linear with period 101 (7 i + 13 j mod 101):
+    # check the mean against the bicubic form instead of a closed form
+    assert abs(float(va_lin.mean()) - float(modes("from_cub_2", (nx, ny)).mean())) < 0.02
+    assert va_lin.shape == (ny, nx) and (sx, sy) == (45, 51)
+
+
+# ------------------------------------------------------------ tile storages
+
+
+@pytest.fixture(scope="module")
+def storage(tmp_path_factory):
+    d = tmp_path_factory.mktemp("vas")
+    r = subprocess.run([DRIVER, "storage", str(d)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(os.path.join(d, "storage.txt")).read().split("\n")
+
+    def load(name):
+        return np.fromfile(os.path.join(d, name + ".f32"), dtype=np.float32).reshape(97, 130)
+
+    return lines, load
+
+
+def _storage_sequence(smooth):
+    va = V.VA((130, 97), (-1.5, 2.25, 0.1, 0.9), (40, 33), 5)
+    va.from_array(pattern(130, 97))
+    for r in va.regions():
+        ii = np.arange(r.shape[0])[None, :]
+        jj = np.arange(r.shape[1])[:, None]
+        t = va.tile(r)
+        t[...] = (t + F(r.key[0] * 3 + r.key[1] * 5)) + ((ii + 2 * jj) % 7).astype(np.float32) / F(7)
+    if smooth:
+        va.smooth_overlap_buffers()
+    va.remap(0.0, 1.0)
+    va.set(7, 9, 2.5)
+    a = va.to_array()
+    for r in va.regions():
+        va.tile(r)[...] = F(2) * va.tile(r) + F(1)
+    return a, va.to_array(), va.sum()
+
+
+MODES = ("VA_RAM", "VA_DISK_LRU", "VA_DISK_LRU_MIN", "VA_DISK_SEQUENTIAL")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_storages_give_the_grid_of_the_restatement(storage, mode):
+    """Every store gives the RAM grid; the sequential store holds one tile,
+    so overlap smoothing is skipped there (as the reference's does)."""
+    lines, load = storage
+    a, b, bsum = _storage_sequence(smooth=mode != "VA_DISK_SEQUENTIAL")
+    assert np.array_equal(load("st_a_" + mode), a)
+    assert np.array_equal(load("st_b_" + mode), b)
+    row = [ln for ln in lines if ln.startswith("mode %s " % mode)][0]
+    head, info = row.split("|")
+    _, _, live, got, s = head.split()
+    assert hexf(got) == F(2.5) and hexf(s) == bsum
+    want_live = {"VA_RAM": 2 ** 64 - 1, "VA_DISK_LRU": 12, "VA_DISK_LRU_MIN": 2,
+                 "VA_DISK_SEQUENTIAL": 1}[mode]
+    assert int(live) == want_live
+    assert info == {"VA_RAM": "RAM", "VA_DISK_LRU": "DiskLRU (capacity=12)",
+                    "VA_DISK_LRU_MIN": "DiskLRU (capacity=2)",
+                    "VA_DISK_SEQUENTIAL": "DiskSequential"}[mode]
+
+
+def test_storage_rules(storage):
+    lines, _ = storage
+    rows = {ln.split()[0] + (" " + ln.split()[1] if ln.startswith("trim") else ""): ln.split()
+            for ln in lines if ln}
+    # trim writes every tile of the 4 x 3 grid out and frees them
+    for m in ("VA_DISK_LRU", "VA_DISK_LRU_MIN"):
+        assert rows["trim " + m][2:] == ["0", "12"]
+    # tiles are 45 x 38 (0,0) and 50 x 38 ((1,0), (2,0)) cells with halos
+    n00, n10, n20 = 45 * 38, 50 * 38, 50 * 38
+    # RAM LRU of two: the tile that left comes back zero-filled
+    assert rows["lru"][1:] == ["2", float(F(0)).hex().replace("0x0.0p+0", "0x0p+0"),
+                               rows["lru"][3]] and hexf(rows["lru"][3]) == F(3 * n20)
+    # disk LRU of two: the tile that left is on disk and comes back intact
+    r = rows["disklru"]
+    assert r[1:3] == ["2", "1"]
+    assert (hexf(r[3]), hexf(r[4]), hexf(r[5])) == (F(n00), F(2 * n10), F(3 * n20))
+    # three tiles held at once in a store of two; room made down to two
+    r = rows["held"]
+    assert r[1:3] == ["3", "2"] and [hexf(x) for x in r[3:]] == [F(4), F(5), F(6)]
+    assert rows["removed"][1] == "1"
+    # one tile at a time: a second get before the release is refused
+    r = rows["seq"]
+    assert r[1:3] == ["1", "2"] and hexf(r[3]) == F(2 * 50 * 43) and hexf(r[4]) == F(9)
