@@ -281,6 +281,8 @@ def batch_summary(args):
             "per_map_plain_gcell_iter_s": r["per_map_plain_gcell_iter_s"],
             "per_map_fused_gcell_iter_s": r["per_map_fused_gcell_iter_s"],
             "speedup_vs_per_map_plain": r["speedup_vs_per_map_plain"],
+            "graph_gcell_iter_s": r["graph_gcell_iter_s"],
+            "persistent_gcell_iter_s": r["persistent_gcell_iter_s"],
             "launch_shape": r["config"]["tuning"], "storage": args.storage}
 
 
@@ -321,11 +323,16 @@ def served_summary(args, requests=2000, sub_requests=500, sub_rate=40.0):
 # ---------------------------------------------------------------- batch
 
 
+FORMS = {0: "stream", 1: "graph", 2: "persistent"}
+
+
 def run_batch(args):
     """96 maps (seeds 100..195, shapes and iteration counts from the batch
-    manifest): one batched launch per iteration captured in a hipGraph, with
-    the autotuned launch shape, against the same maps run one after another
-    with per-map plain launches (and per-map fused launches)."""
+    manifest) with the autotuned launch shape and form: one batched launch
+    per iteration captured in a hipGraph, or one persistent cooperative
+    launch with a grid barrier between iterations (both are timed; `value`
+    is the form the autotuner picked), against the same maps run one after
+    another with per-map plain launches (and per-map fused launches)."""
     from highmap_amd import workloads
 
     torch, hm, ctx = _gpu()
@@ -337,21 +344,30 @@ def run_batch(args):
     cell_iters = sum(m["nx"] * m["ny"] * m["iterations"] for m in maps)
     st = STORAGE[args.storage]
 
-    # warm-up and autotune on a throw-away copy
+    # warm-up and autotune (shape, then graph against persistent launch) on
+    # a throw-away copy: 15 shapes and 2 forms of 2 probe iterations each
     b = hm.Batch(storage=st, **WORKLOAD)
     for z, m in zip(zs, maps):
-        b.add(z, 24)
+        b.add(z, 40)
     tuned = b.autotune(probe=2)
+    form = b.tuned_form()
     b.close()
 
-    b = hm.Batch(storage=st, **WORKLOAD)
-    for z, m in zip(zs, maps):
-        b.add(z, m["iterations"])
-    b.set_tuning(*tuned)
-    t0 = time.perf_counter()
-    dev_ms = b.run(use_graph=True)
-    batched_s = time.perf_counter() - t0
-    b.close()
+    def batched(f):
+        b = hm.Batch(storage=st, **WORKLOAD)
+        for z, m in zip(zs, maps):
+            b.add(z, m["iterations"])
+        b.set_tuning(*tuned)
+        t0 = time.perf_counter()
+        dev_ms = b.run(form=f)
+        dt = time.perf_counter() - t0
+        b.close()
+        return dt, dev_ms
+
+    graph_s, graph_ms = batched(hm.Batch.GRAPH)
+    persist_s, persist_ms = batched(hm.Batch.PERSISTENT)
+    batched_s, dev_ms = (persist_s, persist_ms) if form == hm.Batch.PERSISTENT else (graph_s,
+                                                                                    graph_ms)
 
     def per_map(mode):
         sess = []
@@ -379,15 +395,64 @@ def run_batch(args):
         "data": "synthetic: remap(noise_fbm PERLIN, kw 4, seed s) per map, s = 100..195",
         "config": {"workload": "batch manifest: %d maps, %d cells, %d cell-iterations"
                                % (len(maps), sum(m["nx"] * m["ny"] for m in maps), cell_iters),
-                   "params": WORKLOAD, "tuning": {"chunk_rows": tuned[0], "prefetch": tuned[1]}},
-        "batched_graph_s": round(batched_s, 4),
+                   "params": WORKLOAD, "tuning": {"chunk_rows": tuned[0], "prefetch": tuned[1],
+                                                  "form": FORMS[form]}},
+        "batched_s": round(batched_s, 4),
         "batched_device_ms": round(dev_ms, 2),
+        "graph_gcell_iter_s": round(cell_iters / graph_s / 1e9, 3),
+        "persistent_gcell_iter_s": round(cell_iters / persist_s / 1e9, 3),
         "per_map_plain_s": round(plain_s, 4),
         "per_map_plain_gcell_iter_s": round(cell_iters / plain_s / 1e9, 3),
         "per_map_fused_s": round(fused_s, 4),
         "per_map_fused_gcell_iter_s": round(cell_iters / fused_s / 1e9, 3),
         "speedup_vs_per_map_plain": round(plain_s / batched_s, 2),
     }
+
+
+def run_batch_forms(args):
+    """Small maps in the three ways to run them: one batch as a hipGraph of
+    per-iteration launches, the same batch as one persistent cooperative
+    launch (grid barrier between iterations), and the maps as sessions one
+    after another.  Batches use the shape the autotuner picks on a
+    throw-away copy; rates from the device time of each batch."""
+    torch, hm, ctx = _gpu()
+    rows = []
+    for n, iters, nmaps in ((1024, 200, 1), (512, 200, 4), (256, 300, 16), (2048, 100, 1)):
+        zs = [device_terrain(torch, hm, ctx, n, n, 2 + k).cpu().numpy() for k in range(nmaps)]
+        b = hm.Batch(**WORKLOAD)
+        for z in zs:
+            b.add(z, 40)
+        tuned = b.autotune(probe=2)
+        form = b.tuned_form()
+        b.close()
+        cells = n * n * nmaps * iters
+        r = {"side": n, "maps": nmaps, "iterations": iters,
+             "tuning": {"chunk_rows": tuned[0], "prefetch": tuned[1], "form": FORMS[form]}}
+        for f in (hm.Batch.GRAPH, hm.Batch.PERSISTENT):
+            b = hm.Batch(**WORKLOAD)
+            b.set_tuning(*tuned)
+            for z in zs:
+                b.add(z, iters)
+            ms = b.run(form=f)
+            b.close()
+            r[FORMS[f] + "_gcell_iter_s"] = round(cells / (ms * 1e-3) / 1e9, 2)
+            r[FORMS[f] + "_us_per_iteration"] = round(ms * 1e3 / iters, 1)
+        sess = [hm.Session(z, **WORKLOAD) for z in zs]
+        ctx.sync()
+        t0 = time.perf_counter()
+        for x in sess:
+            x.vpipes(iters)
+        ctx.sync()
+        r["sessions_gcell_iter_s"] = round(cells / (time.perf_counter() - t0) / 1e9, 2)
+        for x in sess:
+            x.close()
+        print("batchforms %d^2 x %d done" % (n, nmaps), file=sys.stderr, flush=True)
+        rows.append(r)
+    return {"metric": "Gcell-iter/s of small maps: batched graph, persistent, sessions",
+            "unit": "Gcell-iter/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic: remap(noise_fbm PERLIN, kw 4, seed 2 + k)",
+            "config": {"workload": "small maps in batches", "params": WORKLOAD},
+            "results": rows}
 
 
 # ------------------------------------------------------------ precision
@@ -880,8 +945,9 @@ def main():
     ap.add_argument("--max-active-cells", type=int, default=3 * 4096 * 4096,
                     help="serve: cells of the jobs that run at once (0: no cap)")
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "batch", "serve", "precision", "thermal256",
-                             "vpipes1024", "flow", "snow", "tiles", "copy", "ceiling"])
+                    choices=["headline", "batch", "batchforms", "serve", "precision",
+                             "thermal256", "vpipes1024", "flow", "snow", "tiles", "copy",
+                             "ceiling"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline: skip the batch and served-mix measurements")
@@ -895,7 +961,8 @@ def main():
               "multi-GPU path (requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world),
               file=sys.stderr)
         sys.exit(2)
-    fn = {"headline": run_headline, "batch": run_batch, "precision": run_precision,
+    fn = {"headline": run_headline, "batch": run_batch, "batchforms": run_batch_forms,
+          "precision": run_precision,
           "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "flow": run_flow,
           "snow": run_snow, "tiles": run_tiles, "copy": run_copy, "ceiling": run_ceiling}
     if args.workload == "serve":

@@ -154,6 +154,7 @@ def lib():
             "hmg_batch_set_tuning": ([vp, i32, i32], i32),
             "hmg_batch_autotune": ([vp, i32, C.POINTER(i32), C.POINTER(i32)], i32),
             "hmg_batch_run": ([vp, i32, C.POINTER(f32)], i32),
+            "hmg_batch_tuned_form": ([vp, C.POINTER(i32)], i32),
             "hmg_batch_export": ([vp, i32, vp, vp, vp, vp, vp], i32),
             "hmg_batch_destroy": ([vp], i32),
             "hmg_thermal_auto_bedrock": ([vp, vp, i32, i32, vp, f32, i32, vp, vp], i32),
@@ -550,11 +551,22 @@ class Batch:
         _check(lib().hmg_batch_autotune(self.handle, int(probe), C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def run(self, use_graph=True):
-        """Runs every map to its iteration count; returns device ms."""
+    STREAM, GRAPH, PERSISTENT, AUTO = 0, 1, 2, 3  # forms of run (HMG_BATCH_*)
+
+    def run(self, use_graph=True, form=None):
+        """Runs every map to its iteration count; returns device ms.  form
+        (HMG_BATCH_*) overrides use_graph: STREAM, GRAPH, PERSISTENT (one
+        cooperative launch, a grid barrier between iterations) or AUTO (what
+        autotune picked)."""
+        f = int(form) if form is not None else (self.GRAPH if use_graph else self.STREAM)
         ms = C.c_float()
-        _check(lib().hmg_batch_run(self.handle, int(bool(use_graph)), C.byref(ms)))
+        _check(lib().hmg_batch_run(self.handle, f, C.byref(ms)))
         return ms.value
+
+    def tuned_form(self):
+        f = C.c_int()
+        _check(lib().hmg_batch_tuned_form(self.handle, C.byref(f)))
+        return f.value
 
     def export(self, map_id):
         shp = self.shapes[map_id]

@@ -36,6 +36,7 @@
 //   k_vp_export    forms d, s, u, v as hydraulic_vpipes returns them.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -379,7 +380,10 @@ __device__ __forceinline__ int xcd_wave_item()
 // FLOW: the flow and water passes only (flow_simulation): d1 is the stored
 // depth, no transport, erosion or volume sum; only D and F are written
 // (Sp, rain, zo, So and the slot sets are not touched).
-template <class S, bool URAIN, bool PF, bool KEXP1, bool FLOW = false, bool GA = false>
+// NOSYNC: no per-row workgroup barrier (the persistent batch kernel, whose
+// wavefronts of one workgroup may hold items of different lengths).
+template <class S, bool URAIN, bool PF, bool KEXP1, bool FLOW = false, bool GA = false,
+          bool NOSYNC = false>
 __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
                                                const typename S::T *__restrict__ D,
                                                const Flux4<S> *__restrict__ F,
@@ -464,7 +468,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   // step to arrive; six A rows and one row of gathered texels are live.
   static_assert(!GA || (PF && !FLOW), "gathers ahead imply the load-ahead row");
   constexpr int  U = GA ? 6 : (PF ? 5 : 4);
-  constexpr bool ROW_SYNC = !std::is_same<S, StBF16>::value;
+  constexpr bool ROW_SYNC = !NOSYNC && !std::is_same<S, StBF16>::value;
   RowA       R[U];
   float      Sr[U], Cr[U][4];
   TexelFetch TFr[2]; // GA: gathers of the next row to finish, by parity
@@ -480,7 +484,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
     if (!FLOW) Sr[1] = transport(R[0], R[1], R[2], j0);
     flow(R[0], R[1], R[2], Cr[1]);
     if (PF) R[3] = load(j0 + 2);
-    if (GA)
+    if constexpr (GA)
     {
       TFr[0] = transport_issue(R[1], R[2], R[3], j0 + 1);
       R[4] = load(j0 + 3);
@@ -698,16 +702,16 @@ struct VpMapDesc
   int          t_start, it0, iters;
 };
 
-template <class S, bool URAIN, bool PF, bool KEXP1, bool GA = false>
-__global__ void __launch_bounds__(64 * WPB) k_vp_batch(const VpMapDesc *__restrict__ maps,
-                                                  int n_maps,
-                                                  int n_items,
-                                                  int t)
+// Item `item` of batched launch t: finds its map (maps are ordered by
+// item_base) and runs that map's iteration it0 + (t - t_start) on it, if
+// that is inside the map's window.
+template <class S, bool URAIN, bool PF, bool KEXP1, bool GA, bool NOSYNC>
+__device__ __forceinline__ void vp_batch_item(const VpMapDesc *__restrict__ maps,
+                                              int n_maps,
+                                              int item,
+                                              int t)
 {
   using T = typename S::T;
-  const int item = xcd_wave_item();
-  if (item >= n_items) return;
-  // the map holding this item: maps are ordered by item_base
   int lo = 0, hi = n_maps - 1;
   while (lo < hi)
   {
@@ -730,11 +734,112 @@ __global__ void __launch_bounds__(64 * WPB) k_vp_batch(const VpMapDesc *__restri
   Fix128 *prev = m.slots + ((mi + 2) % 3) * NSLOT;
   Fix128 *cur = m.slots + (mi % 3) * NSLOT;
   Fix128 *nxt = m.slots + ((mi + 1) % 3) * NSLOT;
-  vp_stream_item<S, URAIN, PF, KEXP1, false, GA>((const float *)m.z[a], (const T *)m.D[a],
-                                      (const Flux4<S> *)m.F[a], (const T *)m.Sp[a], m.rain,
-                                      (float *)m.z[b], (T *)m.D[b], (Flux4<S> *)m.F[b],
-                                      (T *)m.Sp[b], prev, cur, nxt, item - m.item_base, m.strips,
-                                      m.chunk_rows, m.k, y);
+  vp_stream_item<S, URAIN, PF, KEXP1, false, GA, NOSYNC>(
+      (const float *)m.z[a], (const T *)m.D[a], (const Flux4<S> *)m.F[a], (const T *)m.Sp[a],
+      m.rain, (float *)m.z[b], (T *)m.D[b], (Flux4<S> *)m.F[b], (T *)m.Sp[b], prev, cur, nxt,
+      item - m.item_base, m.strips, m.chunk_rows, m.k, y);
+}
+
+template <class S, bool URAIN, bool PF, bool KEXP1, bool GA = false>
+__global__ void __launch_bounds__(64 * WPB) k_vp_batch(const VpMapDesc *__restrict__ maps,
+                                                  int n_maps,
+                                                  int n_items,
+                                                  int t)
+{
+  const int item = xcd_wave_item();
+  if (item >= n_items) return;
+  vp_batch_item<S, URAIN, PF, KEXP1, GA, false>(maps, n_maps, item, t);
+}
+
+// ------------------------------------------------ persistent batched form
+//
+// One cooperative launch runs launches t = 0 .. T-1 of the batch: every
+// wavefront takes items of launch t, then the whole grid meets at a barrier
+// before launch t + 1 (an iteration reads its neighbours' previous state,
+// and the sediment gathers reach any distance, so no item of t + 1 may start
+// before every item of t is done).  The grid is no larger than what the
+// device holds at once, which the cooperative launch guarantees.
+
+struct GridBar
+{
+  unsigned count; // workgroups arrived at the current barrier
+  unsigned gen;   // barriers completed
+  unsigned err;   // set when a wait gave up
+  unsigned limit; // polls of a wait before it gives up
+};
+
+// polls of a barrier wait (an atomic load and an s_sleep each, about a
+// microsecond) before it gives up and flags the batch as failed: seconds,
+// far beyond one launch's work.  HMG_GRID_SPIN_LIMIT in the environment
+// replaces it (tests force a give-up with a limit of 1).
+constexpr unsigned GRID_SPIN_LIMIT = 1u << 22;
+
+// All workgroups of the grid meet; the writes of each before it are visible
+// to all after it (agent-scope release before arriving, acquire after).
+// Returns false, in every thread of the workgroup, once any wait has given
+// up (the flag is polled while waiting, so the other workgroups stop soon
+// after instead of each waiting out its own limit).
+__device__ __forceinline__ bool grid_barrier(GridBar *bar)
+{
+  __shared__ unsigned ok;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    __threadfence();
+    const unsigned g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned       good = 1u;
+    if (atomicAdd(&bar->count, 1u) == gridDim.x - 1)
+    {
+      __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    else
+    {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+      {
+        if (__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+            ++spins >= bar->limit)
+        {
+          atomicOr(&bar->err, 1u);
+          good = 0u;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __threadfence();
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0u;
+}
+
+// items_at[t], active_at[t]: items and active maps of launch t.  Each XCD
+// (workgroup b runs on XCD b % 8) takes a contiguous eighth of a launch's
+// items, so halo rows shared by neighbouring items stay in its L2.
+template <class S, bool URAIN, bool PF, bool KEXP1, bool GA = false>
+__global__ void __launch_bounds__(64 * WPB) k_vp_batch_persist(const VpMapDesc *__restrict__ maps,
+                                                          const int *__restrict__ items_at,
+                                                          const int *__restrict__ active_at,
+                                                          int      T,
+                                                          GridBar *bar)
+{
+  const int  G = gridDim.x, wave = threadIdx.x >> 6;
+  const bool by_xcd = (G & 7) == 0;
+  const int  parts = by_xcd ? 8 : 1;
+  const int  part = by_xcd ? (int)(blockIdx.x & 7) : 0;
+  const int  slot = __builtin_amdgcn_readfirstlane((by_xcd ? (int)(blockIdx.x >> 3) : (int)blockIdx.x) *
+                                                       WPB + wave);
+  const int  slots = (G / parts) * WPB;
+  for (int t = 0; t < T; t++)
+  {
+    const int n = items_at[t], active = active_at[t];
+    const int i0 = (int)((long)n * part / parts), i1 = (int)((long)n * (part + 1) / parts);
+    for (int item = i0 + slot; item < i1; item += slots)
+      vp_batch_item<S, URAIN, PF, KEXP1, GA, true>(maps, active, item, t);
+    if (!grid_barrier(bar)) return;
+  }
 }
 
 // ---------------------------------------------------------------- export
@@ -1725,6 +1830,82 @@ namespace hmg
 // storage type.  With use_graph the whole launch sequence is captured in a
 // hipGraph and submitted as one graph.  Returns the elapsed device time of
 // the launches in *ms (if not null).
+// The batch's launches t = 0 .. T-1 as one cooperative launch of
+// k_vp_batch_persist (d: the map table in host memory, ordered by iteration
+// count; prefix[k]: items of the first k maps).  Events e0/e1 (may be null)
+// bracket the launch.  Waits for it.  HMG_ERR_UNSUPPORTED: the runtime
+// refused the launch, nothing ran; HMG_ERR_DEVICE: a barrier wait gave up
+// (the maps' state is then not valid).
+template <class S, bool PF, bool KEXP1, bool GA>
+static int batch_persistent(hmg_ctx                      *ctx,
+                            const VpMapDesc              *table,
+                            const std::vector<VpMapDesc> &d,
+                            const std::vector<int>       &prefix,
+                            int                           T,
+                            bool                          any_rain,
+                            hipEvent_t                    e0,
+                            hipEvent_t                    e1)
+{
+  hipStream_t      st = ctx->stream;
+  std::vector<int> sched(2 * (size_t)(T > 0 ? T : 1), 0); // items, then active maps, per t
+  int              active = (int)d.size(), max_items = 0;
+  for (int t = 0; t < T; t++)
+  {
+    while (active > 0 && d[active - 1].iters <= t) active--;
+    sched[t] = prefix[active];
+    sched[T + t] = active;
+    max_items = std::max(max_items, prefix[active]);
+  }
+  if (max_items == 0)
+  {
+    if (e0) HMG_HIP(hipEventRecord(e0, st));
+    if (e1) HMG_HIP(hipEventRecord(e1, st));
+    HMG_HIP(hipStreamSynchronize(st));
+    return HMG_OK;
+  }
+  const void *f = any_rain ? (const void *)k_vp_batch_persist<S, false, PF, KEXP1, GA>
+                           : (const void *)k_vp_batch_persist<S, true, PF, KEXP1, GA>;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64 * WPB, 0) != hipSuccess ||
+      per_cu < 1)
+    return set_error(HMG_ERR_DEVICE, "persistent batch kernel: no resident workgroup");
+  long G = (long)per_cu * ctx->num_cus;
+  const long need = (max_items + WPB - 1) / WPB;
+  if (G > need) G = need;
+  if (G >= 8) G -= G % 8;
+  DevBuf buf;
+  HMG_TRY(buf.get(ctx->pool, sizeof(GridBar) + sizeof(int) * sched.size()));
+  GridBar *bar = buf.as<GridBar>();
+  int     *items_at = (int *)((char *)buf.ptr + sizeof(GridBar));
+  int     *active_at = items_at + T;
+  GridBar  init{0u, 0u, 0u, GRID_SPIN_LIMIT};
+  if (const char *env = std::getenv("HMG_GRID_SPIN_LIMIT"))
+    init.limit = (unsigned)std::max(1L, std::strtol(env, nullptr, 10));
+  HMG_HIP(hipMemcpyAsync(bar, &init, sizeof(GridBar), hipMemcpyHostToDevice, st));
+  HMG_HIP(hipMemcpyAsync(items_at, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice,
+                         st));
+  void *args[] = {(void *)&table, (void *)&items_at, (void *)&active_at, (void *)&T, (void *)&bar};
+  if (e0) HMG_HIP(hipEventRecord(e0, st));
+  const hipError_t e = hipLaunchCooperativeKernel(f, dim3((unsigned)G), dim3(64 * WPB), args, 0, st);
+  if (e != hipSuccess)
+  {
+    // nothing ran: the maps' state is unchanged (a second process on the
+    // device has been refused the cooperative launch this way)
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(st);
+    return set_error(HMG_ERR_UNSUPPORTED, "persistent batch launch refused: %s",
+                     hipGetErrorString(e));
+  }
+  if (e1) HMG_HIP(hipEventRecord(e1, st));
+  GridBar h{};
+  HMG_HIP(hipMemcpyAsync(&h, bar, sizeof(GridBar), hipMemcpyDeviceToHost, st));
+  HMG_HIP(hipStreamSynchronize(st));
+  if (h.err)
+    return set_error(HMG_ERR_DEVICE,
+                     "persistent batch: a grid barrier wait gave up; the maps' state is not valid");
+  return HMG_OK;
+}
+
 template <class S, bool PF, bool KEXP1, bool GA = false>
 static int batch_advance_t(hmg_ctx           *ctx,
                            hmg_session *const *ss,
@@ -1781,6 +1962,23 @@ static int batch_advance_t(hmg_ctx           *ctx,
   {
     HMG_HIP(hipEventCreate(&e0));
     HMG_HIP(hipEventCreate(&e1));
+  }
+  if (use_graph == HMG_BATCH_PERSISTENT)
+  {
+    HMG_TRY((batch_persistent<S, PF, KEXP1, GA>(ctx, table.as<VpMapDesc>(), d, prefix, T,
+                                                any_rain, e0, e1)));
+    if (ms)
+    {
+      HMG_HIP(hipEventElapsedTime(ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+    for (int i = 0; i < n; i++)
+    {
+      ss[i]->iter += iters[i];
+      ss[i]->cur = (int)(ss[i]->iter & 1);
+    }
+    return HMG_OK;
   }
   hipGraph_t     graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1885,12 +2083,16 @@ int batch_advance(hmg_ctx           *ctx,
 
 } // namespace hmg
 
-// Autotuner cache: best (chunk_rows, prefetch) per (storage, size class),
-// size class = floor(log2(total cells of the batch)).
+// Autotuner cache: best (chunk_rows, prefetch, form) per (storage, size
+// class), size class = floor(log2(total cells of the batch)).
 namespace
 {
-std::mutex                                     g_tune_mu;
-std::map<std::pair<int, int>, std::pair<int, int>> g_tune;
+struct BatchTune
+{
+  int rows, prefetch, form;
+};
+std::mutex                                g_tune_mu;
+std::map<std::pair<int, int>, BatchTune> g_tune;
 } // namespace
 
 struct hmg_batch
@@ -1902,6 +2104,7 @@ struct hmg_batch
   std::vector<int>          iters;
   int                       chunk_rows = 48;
   int                       prefetch = 1;
+  int                       form = HMG_BATCH_GRAPH; // what HMG_BATCH_AUTO runs
   float                     last_ms = 0.f;
 };
 
@@ -1984,8 +2187,9 @@ int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefet
     auto                        it = g_tune.find(key);
     if (it != g_tune.end())
     {
-      b->chunk_rows = it->second.first;
-      b->prefetch = it->second.second;
+      b->chunk_rows = it->second.rows;
+      b->prefetch = it->second.prefetch;
+      b->form = it->second.form;
       if (chunk_rows_out) *chunk_rows_out = b->chunk_rows;
       if (prefetch_out) *prefetch_out = b->prefetch;
       return HMG_OK;
@@ -1996,32 +2200,70 @@ int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefet
   float            best = 1e30f;
   int              best_rows = b->chunk_rows, best_pf = b->prefetch;
   std::vector<int> step(b->maps.size());
+  // runs `probe` iterations of every map in one form and shape; false when
+  // a map has fewer iterations left (every map's count stays exact)
+  auto probe_run = [&](int form, int rows, int pf, float &ms, bool &ran) -> int
+  {
+    ran = false;
+    int min_left = 1 << 30;
+    for (int it : b->iters) min_left = it < min_left ? it : min_left;
+    if (min_left < probe) return HMG_OK;
+    for (size_t i = 0; i < step.size(); i++) step[i] = probe;
+    HMG_TRY(batch_advance(b->ctx, b->maps.data(), step.data(), (int)step.size(), form, rows, pf,
+                          &ms));
+    for (size_t i = 0; i < step.size(); i++) b->iters[i] -= probe;
+    ran = true;
+    return HMG_OK;
+  };
   for (int pf = 0; pf < 3; pf++) // rows loaded when needed, one ahead, gathers ahead
     for (int rows : cand_rows)
     {
-      int min_left = 1 << 30;
-      for (int it : b->iters) min_left = it < min_left ? it : min_left;
-      if (min_left < probe) break; // keep every map's iteration count exact
-      for (size_t i = 0; i < step.size(); i++) step[i] = probe;
       float ms = 0.f;
-      HMG_TRY(batch_advance(b->ctx, b->maps.data(), step.data(), (int)step.size(), 0, rows, pf,
-                            &ms));
-      for (size_t i = 0; i < step.size(); i++) b->iters[i] -= probe;
-      if (ms < best)
+      bool  ran = false;
+      HMG_TRY(probe_run(HMG_BATCH_STREAM, rows, pf, ms, ran));
+      if (ran && ms < best)
       {
         best = ms;
         best_rows = rows;
         best_pf = pf;
       }
     }
+  // the form, with the chosen shape: the captured launch sequence against
+  // the persistent launch
+  int   best_form = HMG_BATCH_GRAPH;
+  float ms_graph = 0.f, ms_persist = 0.f;
+  bool  ran_graph = false, ran_persist = false;
+  HMG_TRY(probe_run(HMG_BATCH_GRAPH, best_rows, best_pf, ms_graph, ran_graph));
+  {
+    // a refused cooperative launch leaves the maps as they were: keep the
+    // graph form
+    const int rc = probe_run(HMG_BATCH_PERSISTENT, best_rows, best_pf, ms_persist, ran_persist);
+    if (rc == HMG_ERR_UNSUPPORTED)
+    {
+      clear_error();
+      ran_persist = false;
+    }
+    else
+      HMG_TRY(rc);
+  }
+  if (ran_graph && ran_persist && ms_persist < ms_graph) best_form = HMG_BATCH_PERSISTENT;
   b->chunk_rows = best_rows;
   b->prefetch = best_pf;
+  b->form = best_form;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[key] = std::make_pair(best_rows, best_pf);
+    g_tune[key] = BatchTune{best_rows, best_pf, best_form};
   }
   if (chunk_rows_out) *chunk_rows_out = best_rows;
   if (prefetch_out) *prefetch_out = best_pf;
+  return HMG_OK;
+}
+
+int hmg_batch_tuned_form(hmg_batch *b, int *form)
+{
+  clear_error();
+  if (!b || !form) return set_error(HMG_ERR_INVALID, "null argument");
+  *form = b->form;
   return HMG_OK;
 }
 
@@ -2031,8 +2273,12 @@ int hmg_batch_run(hmg_batch *b, int use_graph, float *device_ms)
   if (!b) return set_error(HMG_ERR_INVALID, "null batch");
   std::lock_guard<std::recursive_mutex> lk(b->ctx->mu);
   HMG_TRY(ctx_enter(b->ctx));
-  float ms = 0.f;
-  HMG_TRY(batch_advance(b->ctx, b->maps.data(), b->iters.data(), (int)b->maps.size(), use_graph,
+  float     ms = 0.f;
+  const int form = use_graph == HMG_BATCH_AUTO ? b->form
+                   : use_graph == HMG_BATCH_PERSISTENT ? HMG_BATCH_PERSISTENT
+                   : use_graph ? HMG_BATCH_GRAPH
+                               : HMG_BATCH_STREAM;
+  HMG_TRY(batch_advance(b->ctx, b->maps.data(), b->iters.data(), (int)b->maps.size(), form,
                         b->chunk_rows, b->prefetch, &ms));
   for (int &it : b->iters) it = 0;
   b->last_ms = ms;

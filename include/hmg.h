@@ -500,12 +500,34 @@ int hmg_batch_add(hmg_batch   *b,
 int hmg_batch_set_tuning(hmg_batch *b, int chunk_rows, int prefetch);
 /* Picks the launch shape by timing `probe` iterations of each candidate on
  * the live maps (they count toward each map's iterations; results do not
- * depend on the shape); caches the choice per storage type and batch size. */
+ * depend on the shape), then the form (graph or persistent, `probe`
+ * iterations each with that shape); caches the choice per storage type and
+ * batch size. */
 int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefetch_out);
-/* Runs every map to its iteration count.  use_graph captures the launch
- * sequence in one hipGraph.  *device_ms (may be NULL) receives the elapsed
- * device time of the launches. */
+/* Forms of hmg_batch_run. */
+#define HMG_BATCH_STREAM 0     /* one launch per iteration on the stream */
+#define HMG_BATCH_GRAPH 1      /* the same launches captured in one hipGraph */
+#define HMG_BATCH_PERSISTENT 2 /* one cooperative launch for all iterations: a
+                                  grid no larger than the device holds at once,
+                                  every wavefront looping over the items of
+                                  each iteration, a grid-wide barrier between
+                                  iterations (the sediment gathers reach any
+                                  distance, so an iteration needs all of the
+                                  previous one); no per-row workgroup barrier */
+#define HMG_BATCH_AUTO 3       /* the form hmg_batch_autotune picked (graph
+                                  until it has run) */
+/* Runs every map to its iteration count in the given form (above; for
+ * compatibility any other nonzero value means HMG_BATCH_GRAPH).
+ * *device_ms (may be NULL) receives the elapsed device time of the
+ * launches.  A persistent run the runtime refuses (observed when another
+ * process on the device has made cooperative launches) returns
+ * HMG_ERR_UNSUPPORTED with the maps unchanged; one whose barrier wait gave
+ * up (seconds without progress) returns HMG_ERR_DEVICE and leaves the maps'
+ * state invalid. */
 int hmg_batch_run(hmg_batch *b, int use_graph, float *device_ms);
+/* The form hmg_batch_autotune picked (HMG_BATCH_GRAPH or
+ * HMG_BATCH_PERSISTENT) into *form. */
+int hmg_batch_tuned_form(hmg_batch *b, int *form);
 /* Copies one map's state out as hmg_session_export does. */
 int hmg_batch_export(hmg_batch *b, int map_id, float *z, float *d, float *s, float *u, float *v);
 int hmg_batch_destroy(hmg_batch *b);

@@ -9,8 +9,9 @@ bit for bit (every plane the drop-in returns: z, d, s, u, v).
     hydraulic iteration, one continuous state): FP32 for 500 iterations,
     BF16 and FP16 storage for 80 (the online tuner probes iterations 20-55,
     so the tuned shape runs after that);
-  - the 96-map batch manifest: every map batched equals the map alone, and
-    12 maps chosen across sizes and shapes equal the oracle;
+  - the 96-map batch manifest: every map batched equals the map alone (the
+    persistent launch gives the same maps), and 12 maps chosen across sizes
+    and shapes equal the oracle;
   - the first 40 requests of the served Poisson stream at their real sides
     (256-3456) with BF16 state: served equals alone for all, and five jobs
     of both kinds equal the oracle.
@@ -172,9 +173,17 @@ def test_batch_manifest_96_maps(hm, orc):
     zs = [orc.fbm_terrain(m["nx"], m["ny"], m["seed"]) for m in maps]
     b = hm.Batch(**VP_WORKLOAD)
     ids = [b.add(z, m["iterations"]) for z, m in zip(zs, maps)]
-    b.autotune(probe=2)
+    rows, pf = b.autotune(probe=2)
     b.run(use_graph=True)
     outs = [b.export(i) for i in ids]
+    b.close()
+    # the same batch as one persistent launch
+    b = hm.Batch(**VP_WORKLOAD)
+    b.set_tuning(rows, pf)
+    ids = [b.add(z, m["iterations"]) for z, m in zip(zs, maps)]
+    b.run(form=hm.Batch.PERSISTENT)
+    for k, i in enumerate(ids):
+        _assert_same(b.export(i), outs[k], "persistent map %d" % k)
     b.close()
     for k, (z, m) in enumerate(zip(zs, maps)):
         s = hm.Session(z, **VP_WORKLOAD)
