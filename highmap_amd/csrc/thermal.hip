@@ -380,6 +380,75 @@ __device__ __forceinline__ int th_wave_item()
   return __builtin_amdgcn_readfirstlane(b * 4 + (threadIdx.x >> 6));
 }
 
+// The exchanges of a row (z0) with the row above it (z1): axial and both
+// diagonals, the diagonals with the IEEE division for the whole wave when a
+// lane holds a quotient outside div_1414's range.
+__device__ __forceinline__ void th_row_pairs(
+    float z0, float z1, float md, float mdd, float &eU, float &eUR, float &eUL)
+{
+  bool odd = false;
+  eU = th_pair_axial(z0, z1, md);
+  eUR = th_pair_diag<false>(z0, th_right(z1), mdd, odd);
+  eUL = th_pair_diag<false>(z0, th_left(z1), mdd, odd);
+  if (__builtin_expect(__any(odd), 0))
+  {
+    eUR = th_pair_diag<true>(z0, th_right(z1), mdd, odd);
+    eUL = th_pair_diag<true>(z0, th_left(z1), mdd, odd);
+  }
+}
+
+// One Jacobi step of a row before the border rule: z0, z1 the row and the
+// row above it in the step's input; pU/pUR/pUL the exchanges of the row
+// below with this one, replaced by this row's with the row above.  The
+// eight terms in the reference's order (see above).
+__device__ __forceinline__ float th_row_step(
+    float z0, float z1, float md, float mdd, float &pU, float &pUR, float &pUL)
+{
+  float eU, eUR, eUL;
+  th_row_pairs(z0, z1, md, mdd, eU, eUR, eUL);
+  const float eR = th_pair_axial(z0, th_right(z0), md);
+  // (-1, 0) as 0 - term: the empty asm keeps the compiler from folding
+  // (0 - a) + b into b - a, which differs from it in the sign of a zero
+  float amount = 0.f - th_left(eR); // (-1, 0)
+  asm volatile("" : "+v"(amount));
+  amount = amount + eU;             // (0, 1)
+  amount = amount - pU;             // (0, -1)
+  amount = amount + eR;             // (1, 0)
+  amount = amount - th_left(pUR);   // (-1, -1)
+  amount = amount + eUL;            // (-1, 1)
+  amount = amount - th_right(pUL);  // (1, -1)
+  amount = amount + eUR;            // (1, 1)
+  float step = z0 + amount;
+  // formed in every lane, before any EXEC mask of the caller (the lane
+  // exchanges above then fold into the additions)
+  asm volatile("" : "+v"(step));
+  pU = eU;
+  pUR = eUR;
+  pUL = eUL;
+  return step;
+}
+
+// apply_boundaries (_common_index.cl:106-132) on row r of a step's output:
+// i == 0, i == nx - 1, j == 0, j == ny - 1 in that order copy the inward
+// neighbour of the step's input (zm, z0, z1: rows r-1, r, r+1).  Strips
+// away from columns 0 and nx - 1 (edge false) skip the column rules.  The
+// lane exchange runs in every lane before the selects (a DPP read from a
+// lane the EXEC mask disables gives 0).
+__device__ __forceinline__ float th_border(
+    float out, float zm, float z0, float z1, int r, int ny, bool edge, bool c0, bool cn)
+{
+  if (edge || r == 0 || r == ny - 1)
+  {
+    float zl = th_left(z0), zr = th_right(z0);
+    asm volatile("" : "+v"(zl), "+v"(zr));
+    if (r == 0) out = z1;
+    if (r == ny - 1) out = zm;
+    out = cn ? zl : out;
+    out = c0 ? zr : out;
+  }
+  return out;
+}
+
 template <int P, bool EXTRAP>
 __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict__ z,
                                                         float *__restrict__ zn,
@@ -424,17 +493,7 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
 #pragma unroll
   for (int q = 0; q <= P + 1; q++) R[q] = load(j0 - 1 + q);
   float pU, pUR, pUL; // exchanges of the previous row with the row above it
-  {
-    bool odd = false;
-    pU = th_pair_axial(R[0], R[1], md);
-    pUR = th_pair_diag<false>(R[0], th_right(R[1]), mdd, odd);
-    pUL = th_pair_diag<false>(R[0], th_left(R[1]), mdd, odd);
-    if (__builtin_expect(__any(odd), 0))
-    {
-      pUR = th_pair_diag<true>(R[0], th_right(R[1]), mdd, odd);
-      pUL = th_pair_diag<true>(R[0], th_left(R[1]), mdd, odd);
-    }
-  }
+  th_row_pairs(R[0], R[1], md, mdd, pU, pUR, pUL);
   float f1 = 0.f, f2 = 0.f; // EXTRAP: final values of rows r-1 and r-2
 
   for (int r0 = j0; r0 < j1; r0 += U)
@@ -448,52 +507,12 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
       // the row P + 1 below the next one, into the slot of row r - 1
       if (r + P + 1 <= j1) R[(q + P + 2) % U] = load(r + P + 1);
 
-      bool  odd = false;
-      float eR = th_pair_axial(z0, th_right(z0), md);
-      float eU = th_pair_axial(z0, z1, md);
-      float eUR = th_pair_diag<false>(z0, th_right(z1), mdd, odd);
-      float eUL = th_pair_diag<false>(z0, th_left(z1), mdd, odd);
-      if (__builtin_expect(__any(odd), 0))
-      {
-        eUR = th_pair_diag<true>(z0, th_right(z1), mdd, odd);
-        eUL = th_pair_diag<true>(z0, th_left(z1), mdd, odd);
-      }
-      // (-1, 0) as 0 - term: the empty asm keeps the compiler from folding
-      // (0 - a) + b into b - a, which differs from it in the sign of a zero
-      float amount = 0.f - th_left(eR); // (-1, 0)
-      asm volatile("" : "+v"(amount));
-      amount = amount + eU;             // (0, 1)
-      amount = amount - pU;             // (0, -1)
-      amount = amount + eR;             // (1, 0)
-      amount = amount - th_left(pUR);   // (-1, -1)
-      amount = amount + eUL;            // (-1, 1)
-      amount = amount - th_right(pUL);  // (1, -1)
-      amount = amount + eUR;            // (1, 1)
-      float step = z0 + amount;
-      // formed in every lane, before the store's EXEC mask (the lane
-      // exchanges above then fold into the additions)
-      asm volatile("" : "+v"(step));
-      pU = eU;
-      pUR = eUR;
-      pUL = eUL;
+      const float step = th_row_step(z0, z1, md, mdd, pU, pUR, pUL);
 
       float *row = zn + (size_t)r * (size_t)nx;
       if (!EXTRAP)
       {
-        float out = step;
-        if (edge_strip || r == 0 || r == ny - 1)
-        {
-          // apply_boundaries (_common_index.cl:106-132): i == 0, i == nx - 1,
-          // j == 0, j == ny - 1 in that order copy the inward neighbour.
-          // The lane exchange runs in every lane before the selects (a DPP
-          // read from a lane the EXEC mask disables gives 0).
-          float zl = th_left(z0), zr = th_right(z0);
-          asm volatile("" : "+v"(zl), "+v"(zr));
-          if (r == 0) out = z1;
-          if (r == ny - 1) out = zm;
-          out = cn ? zl : out;
-          out = c0 ? zr : out;
-        }
+        const float out = th_border(step, zm, z0, z1, r, ny, edge_strip, c0, cn);
         if (writer) at_off(row, soff) = out;
       }
       else
@@ -521,6 +540,120 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
 
 // Launch geometry of k_thermal_stream: strips of 62 columns and chunks sized
 // so that the work items just fill the device's wavefront slots (one round).
+// ------------------------------------------------- two steps per launch
+//
+// k_thermal_stream2: two Jacobi steps with a uniform talus in one launch.
+// Each wavefront streams its rows once: step 1 of row r from the input rows
+// r-1..r+1 (loaded P rows ahead), then step 2 of row r-1 from step 1's
+// rows r-2..r, held in registers, so a step costs half a read and half a
+// write per cell.  Step 1 is valid in lanes 1..62 and step 2 in lanes
+// 2..61: a strip writes 60 columns (the last strip moved to end at column
+// nx - 1).  Each step applies the border rule to its own output, so step 2
+// reads exactly the field that a separate launch would have written.  Rows
+// above row 0 and below row ny - 1 come from clamped loads; they reach only
+// values that the border rule replaces.  A chunk of rows j0..j1-1 runs step
+// 1 on rows j0-1..j1.
+
+constexpr int TH2_OUT = 60; // columns written per wavefront
+
+template <int P>
+__global__ void __launch_bounds__(256) k_thermal_stream2(const float *__restrict__ z,
+                                                         float *__restrict__ zn,
+                                                         int   nx,
+                                                         int   ny,
+                                                         float ts,
+                                                         int   strips,
+                                                         int   chunk_rows,
+                                                         int   n_items)
+{
+  const int item = th_wave_item();
+  if (item >= n_items) return;
+  const int      lane = threadIdx.x & 63;
+  const int      strip = item % strips, chunk = item / strips;
+  const int      x0 = nx > TH2_OUT ? min(strip * TH2_OUT - 2, nx - TH2_OUT - 2) : -2;
+  const int      gx = x0 + lane;
+  const uint32_t off = (uint32_t)min(max(gx, 0), nx - 1); // column loaded
+  const uint32_t soff = (uint32_t)max(gx, 0);              // column stored
+  const bool     writer = lane >= 2 && lane < 2 + TH2_OUT && gx < nx;
+  const bool     c0 = gx == 0, cn = gx == nx - 1;
+  const bool     edge_strip = x0 < 0 || x0 + TH2_OUT + 2 >= nx;
+  const int      j0 = chunk * chunk_rows;
+  const int      j1 = min(j0 + chunk_rows, ny);
+  const float    md = ts, mdd = 1.414f * ts;
+  auto           load = [&](int r) {
+    const float *row = z + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
+    return at_off(row, off);
+  };
+
+  // rings of U slots indexed by compile-time constants (rows rotate by
+  // renaming): with step-1 row r, Z holds input rows r-1 .. r+P+1 and S1
+  // step-1 rows r-2 .. r
+  constexpr int U = P + 3;
+  float         Z[U], S1[U];
+  float         pU, pUR, pUL; // step 1: exchanges of input row r-1 with row r
+  float         qU, qUR, qUL; // step 2: exchanges of step-1 row r-2 with row r-1
+  {
+    // pre-roll: step 1 of rows j0-1 and j0, step 2's exchanges between them
+    const float za = load(j0 - 2), zb = load(j0 - 1);
+#pragma unroll
+    for (int q = 0; q <= P + 1; q++) Z[q] = load(j0 + q);
+    th_row_pairs(za, zb, md, mdd, pU, pUR, pUL);
+    float s = th_row_step(zb, Z[0], md, mdd, pU, pUR, pUL);
+    S1[0] = th_border(s, za, zb, Z[0], j0 - 1, ny, edge_strip, c0, cn);
+    s = th_row_step(Z[0], Z[1], md, mdd, pU, pUR, pUL);
+    S1[1] = th_border(s, zb, Z[0], Z[1], j0, ny, edge_strip, c0, cn);
+    th_row_pairs(S1[0], S1[1], md, mdd, qU, qUR, qUL);
+  }
+
+  for (int r0 = j0 + 1; r0 <= j1; r0 += U)
+  {
+#pragma unroll
+    for (int q = 0; q < U; q++)
+    {
+      const int r = r0 + q;
+      if (r > j1) break;
+      const float zm = Z[q % U], z0 = Z[(q + 1) % U], z1 = Z[(q + 2) % U];
+      // input row r+P+1 into the slot of row r-2 (the last needed is j1+1)
+      if (r + P + 1 <= j1 + 1) Z[(q + P + 2) % U] = load(r + P + 1);
+      // step 1 of row r
+      float s = th_row_step(z0, z1, md, mdd, pU, pUR, pUL);
+      s = th_border(s, zm, z0, z1, r, ny, edge_strip, c0, cn);
+      S1[(q + 2) % U] = s;
+      // step 2 of row r-1
+      const float a = S1[q % U], b = S1[(q + 1) % U];
+      float       o = th_row_step(b, s, md, mdd, qU, qUR, qUL);
+      o = th_border(o, a, b, s, r - 1, ny, edge_strip, c0, cn);
+      if (writer) at_off(zn + (size_t)(r - 1) * (size_t)nx, soff) = o;
+    }
+  }
+}
+
+template <int P>
+static int launch_thermal_stream2(const float *a, float *b, int nx, int ny, float ts, int num_cus,
+                                  hipStream_t st)
+{
+  static const int blocks_per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream2<P>, 256, 0) !=
+            hipSuccess ||
+        n < 1)
+    {
+      (void)hipGetLastError();
+      n = 2;
+    }
+    return n;
+  }();
+  const long strips = nx > TH2_OUT ? (nx + TH2_OUT - 1) / TH2_OUT : 1;
+  const long slots = 2 * 4L * blocks_per_cu * (num_cus > 0 ? num_cus : 256);
+  long       ch = (strips * ny + slots - 1) / slots;
+  ch = ch < 3 ? 3 : ch;
+  const long chunks = (ny + ch - 1) / ch;
+  const long items = strips * chunks;
+  const int  G = (int)((items + 3) / 4);
+  k_thermal_stream2<P><<<G, 256, 0, st>>>(a, b, nx, ny, ts, (int)strips, (int)ch, (int)items);
+  return check_launch("thermal_stream2");
+}
+
 template <int P, bool EXTRAP>
 static int launch_thermal_stream(const float *a, float *b, int nx, int ny, float ts, int num_cus,
                                  hipStream_t st)
@@ -550,8 +683,10 @@ static int launch_thermal_stream(const float *a, float *b, int nx, int ny, float
   return check_launch("thermal_stream");
 }
 
-// Temporal-blocking depth used for uniform talus below 2^18 cells; 0
-// selects one step per launch (the streaming kernel) at every size.
+// Kernel choice for uniform talus: 8, the LDS-blocked kernel (8 steps per
+// launch) below 2^18 cells and the streaming kernels, two steps per launch,
+// from there; 2, the streaming kernels, two steps per launch, at every
+// size; 0, the one-step streaming kernel at every size.
 static int g_thermal_block_steps = 8;
 
 // Runs `iterations` Jacobi steps, then (extrapolate) extrapolate_borders.
@@ -576,8 +711,8 @@ int launch_thermal(hmg_ctx     *ctx,
   const int T = g_thermal_block_steps;
   const int cus = ctx && ctx->num_cus > 0 ? ctx->num_cus : 256;
   // The blocked kernel recomputes an 8-cell halo per tile; it pays where
-  // launches dominate (below 2^18 cells).  From there the streaming kernel
-  // takes one step per launch.
+  // launches dominate (below 2^18 cells).  From there the streaming kernels
+  // take two steps per launch (one for a last odd step).
   const bool uniform = !talus && !bedrock && talus_scalar >= 0.f;
   const bool blocked_pays = (long)nx * ny < (1L << 18);
   if (uniform && T == 8 && blocked_pays)
@@ -599,6 +734,18 @@ int launch_thermal(hmg_ctx     *ctx,
   const bool stream = uniform;
   dim3       blk(64, 4);
   dim3       grd((nx + 63) / 64, (ny + 3) / 4);
+  // two steps per launch while two or more are left (an odd last step, or
+  // a lone one, runs below; after an even count the borders are
+  // extrapolated by their own launches)
+  const bool pairs = stream && T != 0 && nx >= 4 && ny >= 4;
+  while (pairs && iterations - done >= 2)
+  {
+    HMG_TRY((launch_thermal_stream2<2>(a, b, nx, ny, talus_scalar, cus, st)));
+    float *t = a;
+    a = b;
+    b = t;
+    done += 2;
+  }
   for (; done < iterations; done++)
   {
     const bool last = done == iterations - 1;
@@ -717,7 +864,8 @@ extern "C" {
 
 int hmg_thermal_set_block_steps(int steps)
 {
-  if (steps != 0 && steps != 8) return set_error(HMG_ERR_INVALID, "block steps must be 0 or 8");
+  if (steps != 0 && steps != 2 && steps != 8)
+    return set_error(HMG_ERR_INVALID, "block steps must be 0, 2 or 8");
   g_thermal_block_steps = steps;
   return HMG_OK;
 }

@@ -234,9 +234,10 @@ int hmg_thermal_dev(hmg_ctx     *ctx,
                     const float *bedrock_dev);
 /* Kernel choice of the uniform-talus Jacobi steps: 8 (the default) runs
  * grids below 2^18 cells 8 steps per launch on LDS tiles and larger grids
- * with the one-step streaming kernel; 0 uses the streaming kernel at every
- * size.  Results do not depend on it.  Process-wide; HMG_ERR_INVALID for
- * other values. */
+ * with the streaming kernels, two steps per launch while two are left (the
+ * last odd step alone); 2 uses the streaming kernels that way at every
+ * size; 0 the one-step streaming kernel at every size.  Results do not
+ * depend on it.  Process-wide; HMG_ERR_INVALID for other values. */
 int hmg_thermal_set_block_steps(int steps);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops): the same

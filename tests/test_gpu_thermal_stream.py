@@ -1,7 +1,9 @@
-"""The streaming Jacobi step (uniform talus, one launch per step, the
-thermal() call's extrapolate_borders folded into its last launch) against
-the FP32 oracle, bit for bit (compared as bit patterns, so the sign of a
-zero counts).  With block steps 0 every size takes the streaming kernel;
+"""The streaming Jacobi steps (uniform talus; one step per launch, or two
+per launch with the second step reading the first from registers; the
+thermal() call's extrapolate_borders folded into a last single step)
+against the FP32 oracle, bit for bit (compared as bit patterns, so the sign
+of a zero counts).  With block steps 0 or 2 every size takes the streaming
+kernels;
 the shapes cover one strip (nx <= 62), the strip edge (62, 63, 64, 124,
 125, 126), the moved last strip, grids of 3 to 6 rows, the last chunk
 shorter than 3 rows, zero talus, a talus no slope exceeds and signed
@@ -12,10 +14,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture
-def streaming(hm):
+@pytest.fixture(params=[0, 2], ids=["one_step", "two_steps"])
+def streaming(hm, request):
+    """0: one step per launch at every size; 2: two steps per launch (the
+    last odd step alone, the borders after an even count by their own
+    launches)."""
     L = hm.lib()
-    assert L.hmg_thermal_set_block_steps(0) == 0
+    assert L.hmg_thermal_set_block_steps(request.param) == 0
     yield
     assert L.hmg_thermal_set_block_steps(8) == 0
 
@@ -36,7 +41,7 @@ SHAPES = [(3, 3), (4, 4), (5, 7), (7, 4), (62, 9), (63, 10), (64, 5), (65, 6), (
 
 
 @pytest.mark.parametrize("nx,ny", SHAPES)
-@pytest.mark.parametrize("iters", [1, 2, 5])
+@pytest.mark.parametrize("iters", [1, 2, 4, 5])
 def test_stream_steps_match_oracle(hm, orc, streaming, nx, ny, iters):
     z0 = _input(orc, nx, ny, nx * 1000 + ny)
     for talus in (0.5 / nx, 4.0 / nx):
@@ -88,3 +93,14 @@ def test_stream_in_session_matches_oracle(hm, orc, streaming):
     s.close()
     for a, b in zip(g, (o.z, o.d, o.s, o.u, o.v)):
         assert _bits_equal(a, b)
+
+
+@pytest.mark.parametrize("iters", [2, 4, 7])
+def test_default_choice_above_2_18_cells(hm, orc, iters):
+    """The default kernel choice at 1500 x 1100: two steps per launch, then
+    a last odd step with the borders or, after an even count, the border
+    launches."""
+    z0 = orc.fbm_terrain(1500, 1100, 11)
+    z = z0.copy()
+    hm.thermal(z, 2.0 / 1500, iters)
+    assert _bits_equal(z, orc.thermal(z0, 2.0 / 1500, iters))
