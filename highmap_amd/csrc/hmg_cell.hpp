@@ -19,11 +19,14 @@ namespace hmg
 
 // Storage codecs for the water, sediment and flux planes.  Values are
 // rounded to nearest even when stored; all arithmetic stays FP32.
+// st_rounded(x) is st(x) for an x that rnd<S> returned (already a stored
+// value): the bit pattern is taken as is, with no second rounding.
 struct StF32
 {
   using T = float;
   static __device__ __forceinline__ float ld(T x) { return x; }
   static __device__ __forceinline__ T     st(float x) { return x; }
+  static __device__ __forceinline__ T     st_rounded(float x) { return x; }
 };
 
 struct StBF16
@@ -50,6 +53,8 @@ struct StBF16
     }
     return r;
   }
+  // ld(st(x)) has the 16 stored bits on top and zeros below, NaNs too
+  static __device__ __forceinline__ T st_rounded(float x) { return (T)(__float_as_uint(x) >> 16); }
   // the integer form: add half an ulp (ties to even) and truncate
   static __device__ __forceinline__ T st_int(float x)
   {
@@ -74,6 +79,8 @@ struct StF16
     asm volatile("" : "+v"(x));
     return (_Float16)x;
   }
+  // an FP16 value widened to FP32 converts back exactly
+  static __device__ __forceinline__ T st_rounded(float x) { return st(x); }
 };
 
 // Rounds a value to what the storage type would hold.
@@ -81,6 +88,40 @@ template <class S>
 __device__ __forceinline__ float rnd(float x)
 {
   return S::ld(S::st(x));
+}
+
+// rnd of four values.  BF16: two packed conversions (v_cvt_pk_bf16_f32,
+// round to nearest even as st) and one unordered compare per pair for the
+// NaN test, instead of a conversion and a test per value; a wave with a NaN
+// in any of them takes rnd of each.  Same results as rnd of each.
+template <class S>
+__device__ __forceinline__ void rnd4(float &a, float &b, float &c, float &d)
+{
+  a = rnd<S>(a);
+  b = rnd<S>(b);
+  c = rnd<S>(c);
+  d = rnd<S>(d);
+}
+
+template <>
+__device__ __forceinline__ void rnd4<StBF16>(float &a, float &b, float &c, float &d)
+{
+  if (__builtin_expect(__builtin_isunordered(a, b) || __builtin_isunordered(c, d), 0))
+  {
+    a = rnd<StBF16>(a);
+    b = rnd<StBF16>(b);
+    c = rnd<StBF16>(c);
+    d = rnd<StBF16>(d);
+    return;
+  }
+  typedef float  v2f __attribute__((ext_vector_type(2)));
+  typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+  const uint32_t ab = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){a, b}, v2bf));
+  const uint32_t cd = __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){c, d}, v2bf));
+  a = __uint_as_float(ab << 16);
+  b = __uint_as_float(ab & 0xffff0000u);
+  c = __uint_as_float(cd << 16);
+  d = __uint_as_float(cd & 0xffff0000u);
 }
 
 // Four flux components stored together (fl, fr, ft, fb).

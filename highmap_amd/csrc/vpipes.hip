@@ -450,10 +450,11 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
     float zr = from_right(a.z), dr = from_right(a.d1);
     float l = a.fl, r = a.fr, t = a.ft, bb = a.fb;
     vp_flow(a.z, a.d1, zl, dl, zr, dr, ap.z, ap.d1, am.z, am.d1, l, r, t, bb, k);
-    c[0] = rnd<S>(l);
-    c[1] = rnd<S>(r);
-    c[2] = rnd<S>(t);
-    c[3] = rnd<S>(bb);
+    rnd4<S>(l, r, t, bb);
+    c[0] = l;
+    c[1] = r;
+    c[2] = t;
+    c[3] = bb;
   };
 
   // The window lives in rings of U slots indexed by compile-time constants:
@@ -527,12 +528,12 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
         if (writer)
         {
           const uint32_t c = (uint32_t)(r * nx + gx);
-          st_stream(Do, c, S::st(d2));
+          st_stream(Do, c, S::st_rounded(d2));
           Flux4<S> o;
-          o.l = S::st(c_0[0]);
-          o.r = S::st(c_0[1]);
-          o.t = S::st(c_0[2]);
-          o.b = S::st(c_0[3]);
+          o.l = S::st_rounded(c_0[0]);
+          o.r = S::st_rounded(c_0[1]);
+          o.t = S::st_rounded(c_0[2]);
+          o.b = S::st_rounded(c_0[3]);
           st_stream(Fo, c, o);
         }
         continue;
@@ -596,12 +597,12 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
         // every store (the sediment's too) 229.
         st_stream(zo, c, zn);
         at_off(So, c) = S::st(sn);
-        st_stream(Do, c, S::st(d2));
+        st_stream(Do, c, S::st_rounded(d2));
         Flux4<S> o;
-        o.l = S::st(c_0[0]);
-        o.r = S::st(c_0[1]);
-        o.t = S::st(c_0[2]);
-        o.b = S::st(c_0[3]);
+        o.l = S::st_rounded(c_0[0]);
+        o.r = S::st_rounded(c_0[1]);
+        o.t = S::st_rounded(c_0[2]);
+        o.b = S::st_rounded(c_0[3]);
         st_stream(Fo, c, o);
         if (k.maintain) fix_accumulate(acc, d2 * k.ce);
       }
