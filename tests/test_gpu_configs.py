@@ -103,6 +103,8 @@ def _interleaved(sess, o, iters, talus, checkpoints):
         sess.vpipes(10).thermal(talus, 1)
         o.run(10).thermal_step(talus, 1)
         done += 10
+        if done % 50 == 0:
+            print("%d x %d: %d iterations done" % (o.nx, o.ny, done), flush=True)
         if done in checkpoints:
             yield done
 
