@@ -205,6 +205,8 @@ def run_headline(args):
     max_backtrace = sess.max_backtrace()
     sess.close()
     copy_gbs, copy_variant = measured_copy_gbs(torch)
+    pattern_us = (vp_pattern_us(torch, n, n, args.storage, chunk_rows)
+                  if args.mode == "fused" else None)
 
     kernel = (("k_vp_stream_ga" if prefetch == 2 else "k_vp_stream") if args.mode == "fused"
               else "k_vp_* (plain passes)")
@@ -265,6 +267,15 @@ def run_headline(args):
                             "traffic_frac_of_copy": round(tr / launch_s / 1e9 / copy_gbs, 4)
                             if tr else None,
                             "alg_frac_of_copy": round(achieved / copy_gbs, 4)}
+    if pattern_us:
+        # the same memory operations and decomposition without the erosion
+        # arithmetic (k_vp_pattern): the ceiling of this access pattern
+        out["access_pattern_ceiling"] = {
+            "kernel": "k_vp_pattern (hbm_probe.hip)", "chunk_rows": chunk_rows,
+            "avg_launch_us": round(pattern_us, 2),
+            "alg_gbs": round(alg / (pattern_us * 1e-6) / 1e9, 1),
+            "alg_frac_of_peak": round(alg / (pattern_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel_frac_of_pattern": round(pattern_us / (launch_s * 1e6), 4)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)
     if not args.no_extras:
@@ -821,6 +832,36 @@ def probe_lib():
 STREAM_MODES = {"read": 0, "write": 1, "copy": 2}
 
 
+def vp_pattern_us(torch, nx, ny, storage, chunk_rows, reps=30):
+    """Average launch time (us) of k_vp_pattern: the memory operations of one
+    fused vpipes iteration with the kernel's decomposition (strips of 64
+    lanes, 60 written; chunks of `chunk_rows` rows; four wavefronts per
+    workgroup in row lockstep) and no erosion arithmetic, on planes of the
+    storage's element size."""
+    import ctypes as C
+
+    L = probe_lib()
+    L.hmg_probe_vp_pattern.argtypes = [C.c_int] + [C.c_void_p] * 8 + [C.c_int] * 4 + [
+        C.POINTER(C.c_float)]
+    L.hmg_probe_vp_pattern.restype = C.c_int
+    elem = 4 if storage == "f32" else 2
+    dt = torch.float32 if elem == 4 else torch.float16
+    n = nx * ny
+    z, zo = (torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(2))
+    D, Do, S, So = (torch.zeros(n, dtype=dt, device="cuda") for _ in range(4))
+    F, Fo = (torch.zeros(4 * n, dtype=dt, device="cuda") for _ in range(2))
+    torch.cuda.synchronize()
+    ms = C.c_float()
+    rc = L.hmg_probe_vp_pattern(elem, z.data_ptr(), D.data_ptr(), F.data_ptr(), S.data_ptr(),
+                                zo.data_ptr(), Do.data_ptr(), Fo.data_ptr(), So.data_ptr(), nx, ny,
+                                chunk_rows, reps, C.byref(ms))
+    if rc != 0:
+        raise RuntimeError("hmg_probe_vp_pattern failed (%d)" % rc)
+    del z, zo, D, Do, S, So, F, Fo
+    torch.cuda.empty_cache()
+    return ms.value * 1e3
+
+
 def stream_sweep(torch, gib=4, modes=("read", "write", "copy"), unrolls=(1, 2, 4, 8),
                  nts=(0, 1), blocks_per_cu=(0, 4, 8, 16), reps=3):
     """Times the HBM stream kernels over `gib` GiB buffers (far beyond the
@@ -921,6 +962,16 @@ def run_ceiling(args):
         out[mode] = {"best_gbs": rows[0][0], "best_variant": rows[0][1],
                      "frac_of_peak": round(rows[0][0] / HBM_PEAK_GBS, 4),
                      "all": [[g, v["unroll"], v["nontemporal"], v["grid"]] for g, v in rows]}
+    # the fused vpipes kernel's access pattern at 4096^2, FP32 and 16-bit
+    # state, at the chunk lengths the online tuner picks there
+    n = 4096
+    out["vp_pattern"] = {}
+    for storage in ("f32", "bf16"):
+        for cr in (70, 72):
+            us = vp_pattern_us(torch, n, n, storage, cr)
+            out["vp_pattern"]["%s_%d" % (storage, cr)] = {
+                "avg_launch_us": round(us, 2),
+                "alg_gbs": round(ALG_BYTES[storage] * n * n / (us * 1e-6) / 1e9, 1)}
     out["wall_s"] = round(time.perf_counter() - t0, 2)
     return out
 
