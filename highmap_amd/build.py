@@ -95,7 +95,7 @@ def build(verbose=False):
               os.path.join(CSRC, "va_bench.cpp"), "-o", bench + ".tmp", "-L" + HERE,
               "-lhmap_gpu", "-lhmg", "-Wl,-rpath,$ORIGIN"])
         os.replace(bench + ".tmp", bench)
-        for name in ("bf16_check", "arith_check"):
+        for name in ("bf16_check", "arith_check", "pow_check"):
             chk = os.path.join(ROOT, "tests", "cpp", name)
             _run([_hipcc()] + HIPCC_FLAGS + ["-x", "hip", chk + ".cpp", "-o", chk + ".tmp"])
             os.replace(chk + ".tmp", chk)

@@ -268,6 +268,101 @@ __device__ __forceinline__ float hm_exp(float x)
   return (p * __int_as_float((k1 + 127) << 23)) * __int_as_float((k2 + 127) << 23);
 }
 
+// p * 2^k for k in [-151, 128]: two exact powers of two, so that only the
+// last product rounds (into the subnormal range or to infinity).
+__device__ __forceinline__ float hm_scale2(float p, int k)
+{
+  const int k1 = k / 2, k2 = k - k1;
+  return (p * __int_as_float((k1 + 127) << 23)) * __int_as_float((k2 + 127) << 23);
+}
+
+// Project-written pow, FP32 only (OpenCL's pow under -cl-fast-relaxed-math,
+// which the erosion pass calls for k_discharge_exp != 1,
+// hydraulic_vpipes.cl:167, is the vendor's).  The special cases of C99 pow
+// first; then |x| = m 2^e with m in [sqrt(1/2), sqrt(2)) (subnormals scaled
+// by 2^24 first); ln m = 2 atanh(t) with t = (m - 1) / (m + 1) held as two
+// floats (m - 1 exact, m + 1 split by Fast2Sum, the division's fma residual
+// corrected for the low part) and 2 atanh(t) = 2t + t^3 P(t^2), P the
+// series 2/3 + 2/5 s + ... + 2/15 s^6, summed into two floats; log2 |x| =
+// e + ln m / ln 2 and w = y log2 |x| as two floats (products split exactly
+// by fma, the integer part added by TwoSum); 2^w = 2^k 2^f, k = rint(w),
+// 2^f by the degree-8 Taylor polynomial of exp(f ln 2) with fma, then times
+// 2^k as two exact powers of two.  Every step is one IEEE float operation,
+// so the oracle's copy gives the same bits.  Within 0.85 ulp of pow for
+// |y| <= 10 on a sweep (2.6 ulp at |y| = 128).
+__device__ __forceinline__ float hm_powf(float x, float y)
+{
+  if (y == 0.f) return 1.f;
+  if (x == 1.f) return 1.f;
+  if (x != x || y != y) return x + y;
+  const float ay = fabsf(y);
+  const bool  yint = ay >= 0x1p24f || __builtin_rintf(y) == y;
+  const bool  yodd = yint && ay < 0x1p24f && (((int)y) & 1);
+  const bool  xneg = (__float_as_uint(x) >> 31) != 0;
+  float       ax = fabsf(x);
+  if (xneg && !yint && ax != 0.f && ax != __builtin_inff()) return __builtin_nanf("");
+  if (ay == __builtin_inff())
+    return ax == 1.f ? 1.f : ((ax < 1.f) == (y < 0.f) ? __builtin_inff() : 0.f);
+  const float sgn = (xneg && yodd) ? -1.f : 1.f;
+  if (ax == 0.f) return y > 0.f ? sgn * 0.f : sgn * __builtin_inff();
+  if (ax == __builtin_inff()) return y > 0.f ? sgn * __builtin_inff() : sgn * 0.f;
+
+  int e = 0;
+  if (ax < 0x1p-126f)
+  {
+    ax *= 0x1p24f;
+    e = -24;
+  }
+  uint32_t b = __float_as_uint(ax);
+  e += (int)(b >> 23) - 127;
+  float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
+  if (m >= 0x1.6a09e6p+0f)
+  {
+    m *= 0.5f;
+    e += 1;
+  }
+  // ln m = 2 atanh(t), t = (m - 1) / (m + 1)
+  const float u = m - 1.f;
+  const float vh = m + 1.f;
+  const float vl = m - (vh - 1.f);
+  const float th = u / vh;
+  const float rho = fmaf(-th, vh, u) - th * vl;
+  const float tl = rho / vh;
+  const float s = th * th;
+  float       p = fmaf(s, 0x1.111112p-3f, 0x1.3b13b2p-3f);
+  p = fmaf(s, p, 0x1.745d18p-3f);
+  p = fmaf(s, p, 0x1.c71c72p-3f);
+  p = fmaf(s, p, 0x1.24924ap-2f);
+  p = fmaf(s, p, 0x1.99999ap-2f);
+  p = fmaf(s, p, 0x1.555556p-1f);
+  const float tail = fmaf(th * s, p, 2.f * tl);
+  const float lh = 2.f * th + tail;
+  const float ll = tail - (lh - 2.f * th);
+  // log2 |x| = e + (lh + ll) log2(e)
+  const float ph = lh * 0x1.715476p+0f;
+  const float pl = fmaf(lh, 0x1.715476p+0f, -ph) + (lh * 0x1.4ae0c0p-26f + ll * 0x1.715476p+0f);
+  const float fe = (float)e;
+  const float gh = fe + ph;
+  const float bv = gh - fe;
+  const float gl = ((fe - (gh - bv)) + (ph - bv)) + pl;
+  // w = y log2 |x|
+  const float wh = y * gh;
+  if (wh >= 128.5f) return sgn * __builtin_inff();
+  if (wh <= -151.f) return sgn * 0.f;
+  const float wl = fmaf(y, gh, -wh) + y * gl;
+  const float k = __builtin_rintf(wh);
+  const float f = (wh - k) + wl;
+  float       q = fmaf(f, 0x1.62c022p-20f, 0x1.ffcbfcp-17f);
+  q = fmaf(f, q, 0x1.430912p-13f);
+  q = fmaf(f, q, 0x1.5d87fep-10f);
+  q = fmaf(f, q, 0x1.3b2ab6p-7f);
+  q = fmaf(f, q, 0x1.c6b08ep-5f);
+  q = fmaf(f, q, 0x1.ebfbe0p-3f);
+  q = fmaf(f, q, 0x1.62e430p-1f);
+  q = fmaf(f, q, 1.f);
+  return sgn * hm_scale2(q, (int)k);
+}
+
 // OpenCL fract(): min(x - floor(x), 0x1.fffffep-1f).
 __device__ __forceinline__ float fract_cl(float x)
 {
@@ -531,7 +626,7 @@ __device__ __forceinline__ void vp_erosion(float        zc,
   float speed = sqrt_sumsq(u * u + v * v);
   float depth = fminf(d2 / k.dmax, k.ratio);
   float discharge = depth * speed;
-  float pw = (KEXP1 || k.kexp == 1.f) ? discharge : powf(discharge, k.kexp);
+  float pw = (KEXP1 || k.kexp == 1.f) ? discharge : hm_powf(discharge, k.kexp);
   float capa = k.kc * pw * salpha;
   if (capa > sc)
   {

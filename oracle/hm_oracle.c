@@ -698,9 +698,118 @@ static double orc_hypotd(double x, double y)
   return sqrt(x * x + y * y);
 }
 
-static float orc_powf(float x, float e)
+/* p * 2^k for k in [-151, 128]: two exact powers of two, so that only the
+ * last product rounds (into the subnormal range or to infinity). */
+static float orc_scale2(float p, int k)
 {
-  return e == 1.f ? x : powf(x, e);
+  const int k1 = k / 2, k2 = k - k1;
+  return (p * orc_pow2i(k1)) * orc_pow2i(k2);
+}
+
+/* The contract's pow (k_discharge_exp != 1 in the erosion pass,
+ * hydraulic_vpipes.cl:167).  The reference calls OpenCL pow under
+ * -cl-fast-relaxed-math, whose result is the vendor's; the kernels and this
+ * restatement share one FP32-only function instead:
+ *   - the special cases of C99 pow (F.9.4.4) first;
+ *   - |x| = m 2^e with m in [sqrt(1/2), sqrt(2)) (subnormals scaled by 2^24
+ *     first);
+ *   - ln m = 2 atanh(t), t = (m - 1) / (m + 1): m - 1 is exact, m + 1 is
+ *     held as two floats (Fast2Sum), t as two floats (the division's exact
+ *     fma residual corrected for the low part), and 2 atanh(t) = 2t +
+ *     t^3 P(t^2) with P the series 2/3 + 2/5 s + ... + 2/15 s^6 (|t| <
+ *     0.172, the next term below 2^-41), summed into two floats;
+ *   - log2 |x| = e + ln m / ln 2 and w = y log2 |x| as two floats (products
+ *     split exactly by fma, the integer part added by TwoSum);
+ *   - 2^w = 2^k 2^f with k = rint(w), f = (w - k) + w_lo, 2^f by the
+ *     degree-8 Taylor polynomial of exp(f ln 2) in Horner form with fma,
+ *     times 2^k as two exact powers of two.
+ * Every step is an IEEE float operation (explicit fmaf, correctly rounded
+ * division), so the kernels' copy gives the same bits.  Within about 1 ulp
+ * of pow; pinned against Python's double pow over a sweep. */
+float orc_powf(float x, float y)
+{
+  if (y == 0.f) return 1.f;
+  if (x == 1.f) return 1.f;
+  if (x != x || y != y) return x + y;
+  const float ay = fabsf(y);
+  const int   yint = ay >= 0x1p24f || rintf(y) == y;
+  const int   yodd = yint && ay < 0x1p24f && (((int)y) & 1);
+  const int   xneg = signbit(x) != 0;
+  float       ax = fabsf(x);
+  if (xneg && !yint && ax != 0.f && ax != INFINITY) return NAN;
+  if (ay == INFINITY) return ax == 1.f ? 1.f : ((ax < 1.f) == (y < 0.f) ? INFINITY : 0.f);
+  const float sgn = (xneg && yodd) ? -1.f : 1.f;
+  if (ax == 0.f) return y > 0.f ? sgn * 0.f : sgn * INFINITY;
+  if (ax == INFINITY) return y > 0.f ? sgn * INFINITY : sgn * 0.f;
+
+  int e = 0;
+  if (ax < 0x1p-126f)
+  {
+    ax *= 0x1p24f;
+    e = -24;
+  }
+  uint32_t b;
+  memcpy(&b, &ax, sizeof b);
+  e += (int)(b >> 23) - 127;
+  b = (b & 0x7fffffu) | 0x3f800000u;
+  float m;
+  memcpy(&m, &b, sizeof m);
+  if (m >= 0x1.6a09e6p+0f)
+  {
+    m *= 0.5f;
+    e += 1;
+  }
+  /* ln m = 2 atanh(t), t = (m - 1) / (m + 1) */
+  const float u = m - 1.f;
+  const float vh = m + 1.f;
+  const float vl = m - (vh - 1.f);
+  const float th = u / vh;
+  const float rho = fmaf(-th, vh, u) - th * vl;
+  const float tl = rho / vh;
+  const float s = th * th;
+  float       p = fmaf(s, 0x1.111112p-3f, 0x1.3b13b2p-3f);
+  p = fmaf(s, p, 0x1.745d18p-3f);
+  p = fmaf(s, p, 0x1.c71c72p-3f);
+  p = fmaf(s, p, 0x1.24924ap-2f);
+  p = fmaf(s, p, 0x1.99999ap-2f);
+  p = fmaf(s, p, 0x1.555556p-1f);
+  const float tail = fmaf(th * s, p, 2.f * tl);
+  const float lh = 2.f * th + tail;
+  const float ll = tail - (lh - 2.f * th);
+  /* log2 |x| = e + (lh + ll) log2(e) */
+  const float ph = lh * 0x1.715476p+0f;
+  const float pl = fmaf(lh, 0x1.715476p+0f, -ph) + (lh * 0x1.4ae0c0p-26f + ll * 0x1.715476p+0f);
+  const float fe = (float)e;
+  const float gh = fe + ph;
+  const float bv = gh - fe;
+  const float gl = ((fe - (gh - bv)) + (ph - bv)) + pl;
+  /* w = y log2 |x| */
+  const float wh = y * gh;
+  if (wh >= 128.5f) return sgn * INFINITY;
+  if (wh <= -151.f) return sgn * 0.f;
+  const float wl = fmaf(y, gh, -wh) + y * gl;
+  const float k = rintf(wh);
+  const float f = (wh - k) + wl;
+  float       q = fmaf(f, 0x1.62c022p-20f, 0x1.ffcbfcp-17f);
+  q = fmaf(f, q, 0x1.430912p-13f);
+  q = fmaf(f, q, 0x1.5d87fep-10f);
+  q = fmaf(f, q, 0x1.3b2ab6p-7f);
+  q = fmaf(f, q, 0x1.c6b08ep-5f);
+  q = fmaf(f, q, 0x1.ebfbe0p-3f);
+  q = fmaf(f, q, 0x1.62e430p-1f);
+  q = fmaf(f, q, 1.f);
+  return sgn * orc_scale2(q, (int)k);
+}
+
+/* orc_powf over arrays (the tests' sweeps) */
+void orc_powf_many(const float *x, const float *y, float *out, long n)
+{
+  for (long i = 0; i < n; i++) out[i] = orc_powf(x[i], y[i]);
+}
+
+static float orc_powf_disch(float x, float e)
+{
+  return e == 1.f ? x : orc_powf(x, e);
 }
 
 static double orc_powd(double x, double e)
@@ -750,7 +859,7 @@ static double orc_sum_f32fix(const float *a, int nx, int ny)
 #define R_MIN fminf
 #define R_HYPOT orc_hypotf
 #define R_EXP orc_expf
-#define R_POW orc_powf
+#define R_POW orc_powf_disch
 #define R_VOLUME orc_sum_f32fix
 #define R_STORE orc_store_f32
 #define R_IS_F32 1

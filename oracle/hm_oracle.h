@@ -225,6 +225,9 @@ void orc_flow_simulation_f64(const double *z, int nx, int ny, float water_height
 
 /* exp of the FP32 contract (shared with the kernels). */
 float orc_expf(float x);
+/* the contract's pow (FP32 only, shared with the kernels), see hm_oracle.c */
+float orc_powf(float x, float y);
+void  orc_powf_many(const float *x, const float *y, float *out, long n);
 
 /* hmap::gpu::snow_simulation: the final snow depth into s (nx*ny). */
 void orc_snow_simulation_f32(const float *z, int nx, int ny, float snow_depth,

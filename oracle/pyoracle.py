@@ -149,6 +149,9 @@ def lib():
                                           fl, fl, F64P]
     L.orc_expf.argtypes = [fl]
     L.orc_expf.restype = fl
+    L.orc_powf.argtypes = [fl, fl]
+    L.orc_powf.restype = fl
+    L.orc_powf_many.argtypes = [F32P, F32P, F32P, C.c_long]
     snow_tail = [C.c_int] + [fl] * 7 + [C.c_int, fl]
     L.orc_snow_simulation_f32.argtypes = ([F32P, C.c_int, C.c_int, fl, F32P, F32P, F32P]
                                           + snow_tail + [F32P])
@@ -394,6 +397,21 @@ SNOW_DEFAULTS = dict(dt=0.5, fall_iterations_ratio=1.0, k_snow=0.5, k_visc=0.1, 
 def expf(x):
     """The FP32 contract's exp (shared with the kernels)."""
     return float(lib().orc_expf(float(x)))
+
+
+def powf(x, y):
+    """The FP32 contract's pow (shared with the kernels)."""
+    return float(lib().orc_powf(float(x), float(y)))
+
+
+def powf_many(x, y):
+    """powf elementwise over two float32 arrays of one shape."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    assert x.shape == y.shape
+    out = np.empty_like(x)
+    lib().orc_powf_many(x, y, out, x.size)
+    return out
 
 
 def snow_simulation(z, snow_depth, fall_map, melting_map, talus, iterations, dtype=np.float32,

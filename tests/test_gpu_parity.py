@@ -175,15 +175,15 @@ def test_vpipes_bitexact_variants(hm, orc, mode, variant):
     _assert_same(g, o)
 
 
-def test_vpipes_discharge_exponent_tolerance(hm, orc):
-    """k_discharge_exp != 1 goes through powf, whose last bit may differ
-    between the device and glibc: compared with a tolerance of 1e-4 of the
-    height range after 3 iterations."""
-    z0 = _terrain(orc, 96, 96, 3)
-    g = _vp_gpu(hm, z0, 3, "fused", k_discharge_exp=1.3)
-    o = _vp_oracle(orc, z0, 3, k_discharge_exp=1.3)
-    rng = o[0].max() - o[0].min()
-    assert np.abs(g[0] - o[0]).max() <= 1e-4 * rng
+@pytest.mark.parametrize("kexp", [1.3, 0.7, 2.5])
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_vpipes_discharge_exponent(hm, orc, mode, kexp):
+    """k_discharge_exp != 1 goes through the contract's pow, shared by the
+    kernels and the oracle: bit for bit like every other path."""
+    z0 = _terrain(orc, 96, 80, 3)
+    g = _vp_gpu(hm, z0, 12, mode, k_discharge_exp=kexp, **VP_WORKLOAD)
+    o = _vp_oracle(orc, z0, 12, k_discharge_exp=kexp, **VP_WORKLOAD)
+    _assert_same(g, o)
 
 
 # --------------------------------------------------------------- sessions

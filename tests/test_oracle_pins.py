@@ -1,6 +1,6 @@
-"""Independent pins of two oracle functions whose earlier checks were only
-invariants: the erosion/deposition pass of hydraulic_vpipes and the noise
-hashes hash12f / grad22f.
+"""Independent pins of oracle functions whose other checks are only
+invariants: the erosion/deposition pass of hydraulic_vpipes, the noise
+hashes hash12f / grad22f, and the contract's pow (end of the file).
 
 Erosion pass (HighMap/src/gpu_opencl/kernels/hydraulic_vpipes.cl:149-183):
 on a tilted plane every interior cell sees the same gradient, so the pass
@@ -216,3 +216,91 @@ def test_perlin_slope_at_a_lattice_point_is_its_gradient(seed):
         py = O.lib().orc_base_perlin(cx, cy + eps, fseed, 0, 0)
         assert abs(px / eps - 1.42857 * gx) <= 0.01, (cx, cy, px / eps, gx)
         assert abs(py / eps - 1.42857 * gy) <= 0.01, (cx, cy, py / eps, gy)
+
+
+# -------------------------------------------------------------------- pow
+#
+# The contract's pow (orc_powf, shared with the kernels as hm_powf) for
+# k_discharge_exp != 1 in the erosion pass (hydraulic_vpipes.cl:167, OpenCL
+# pow under -cl-fast-relaxed-math there).  Pinned against Python's float64
+# math.pow, the C99 special cases (F.9.4.4) and exact powers.
+
+
+def _ulps(got, want):
+    """|got - want| in units of the float32 spacing at float32(want)."""
+    w32 = np.float32(want)
+    if w32 == 0.0:
+        return abs(got) / 2.0 ** -149
+    return abs(got - want) / float(np.spacing(np.abs(w32)))
+
+
+@pytest.mark.parametrize("y", [0.5, 1.3, 1.5, 2.0, 2.5, 3.0, 0.1, 0.7, -0.7, -1.0, -2.5, 10.0])
+def test_powf_within_one_ulp_of_pow(y):
+    """Log-uniform x over every float exponent (subnormals included) and
+    uniform x in [0, 4): the result is within 0.9 ulp of the float64 pow (a
+    coefficient of either series or of log2(e) with a wrong digit or
+    exponent moves results by many ulps at some x; that the kernels' copy
+    gives the same bits is tests/test_gpu_arith_check.py's)."""
+    y = float(F(y))
+    rng = np.random.default_rng(int(abs(y) * 1000) + 7)
+    xs = np.concatenate([F(2.0) ** rng.uniform(-149, 127, 4000).astype(F),
+                         rng.uniform(0, 4, 4000).astype(F)])
+    xs = xs[(xs > 0) & np.isfinite(xs)]
+    got = O.powf_many(xs, np.full_like(xs, y))
+    worst = 0.0
+    for x, g in zip(xs.tolist(), got.tolist()):
+        try:
+            want = math.pow(x, y)
+        except OverflowError:
+            want = math.inf
+        if want > float(np.finfo(F).max):
+            assert g == math.inf, (x, y, g)
+            continue
+        worst = max(worst, _ulps(g, want))
+    assert worst <= 0.9, (y, worst)
+
+
+def test_powf_large_exponents():
+    """|y| up to 128: the error grows with |y| (the log's error is scaled by
+    it) and stays within 3 ulp."""
+    rng = np.random.default_rng(3)
+    for y in (40.0, -40.0, 127.9):
+        y = float(F(y))
+        xs = rng.uniform(0.5, 2.0, 3000).astype(F)
+        got = O.powf_many(xs, np.full_like(xs, y))
+        for x, g in zip(xs.tolist(), got.tolist()):
+            want = math.pow(x, y)
+            if want > float(np.finfo(F).max):
+                assert g == math.inf
+            else:
+                assert _ulps(g, want) <= 3.0, (x, y, g, want)
+
+
+def test_powf_exact_powers():
+    for x, y, want in [(2.0, 3.0, 8.0), (2.0, -1.0, 0.5), (4.0, 0.5, 2.0), (0.25, 0.5, 0.5),
+                       (2.0, 10.0, 1024.0), (2.0 ** -100, 0.5, 2.0 ** -50), (8.0, 1.0 / 3.0, None),
+                       (3.0, 2.0, 9.0), (10.0, 2.0, 100.0), (1.5, 2.0, 2.25), (2.0, 127.0, 2.0 ** 127),
+                       (2.0, -149.0, 2.0 ** -149), (0.5, 149.0, 2.0 ** -149)]:
+        g = O.powf(x, y)
+        if want is None:
+            assert abs(g - 2.0) <= float(np.spacing(F(2.0))), g
+        else:
+            assert g == want, (x, y, g)
+
+
+def test_powf_special_cases():
+    inf, nan = math.inf, math.nan
+    P = O.powf
+    assert P(nan, 0.0) == 1.0 and P(nan, -0.0) == 1.0
+    assert P(1.0, nan) == 1.0 and P(1.0, inf) == 1.0 and P(-1.0, inf) == 1.0
+    assert math.isnan(P(nan, 1.5)) and math.isnan(P(2.0, nan))
+    assert math.isnan(P(-2.0, 0.5)) and math.isnan(P(-0.3, 1.3))
+    assert P(-2.0, 3.0) == -8.0 and P(-2.0, 2.0) == 4.0 and P(-2.0, -1.0) == -0.5
+    assert P(0.0, 1.3) == 0.0 and P(0.0, -1.3) == inf
+    assert math.copysign(1.0, P(-0.0, 3.0)) == -1.0 and P(-0.0, -3.0) == -inf
+    assert math.copysign(1.0, P(-0.0, 2.0)) == 1.0 and P(-0.0, -2.0) == inf
+    assert P(inf, 0.5) == inf and P(inf, -0.5) == 0.0
+    assert P(-inf, 3.0) == -inf and P(-inf, 2.0) == inf and math.copysign(1.0, P(-inf, -3.0)) == -1.0
+    assert P(0.5, inf) == 0.0 and P(0.5, -inf) == inf and P(2.0, inf) == inf and P(2.0, -inf) == 0.0
+    assert P(2.0, 128.0) == inf and P(2.0, -150.0) == 0.0 and P(2.0, -149.5) == 2.0 ** -149
+    assert P(1e30, 2.0) == inf and P(1e-30, 2.0) == 0.0
