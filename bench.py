@@ -15,8 +15,12 @@ Each run prints one JSON line.  For the headline, `value` is hydraulic
 cell-iterations per second (nx*ny*iterations / wall time of the timed steps,
 thermal included in the time) with the state already resident in HBM;
 `roofline` is measured on the fused vpipes kernel alone with HIP events on
-the library's stream; `cpu_baseline` times the FP32 CPU restatement
-(oracle/cpu_bench) on a bounded sample of the same workload.
+the library's stream; `access_pattern_ceiling` times the same kernel's
+memory operations alone (k_vp_pattern); `cpu_baseline` times the FP32 CPU
+restatement (oracle/cpu_bench) on a bounded sample of the same workload.
+Unless --no-extras: the kernel's rate with BF16 and FP16 state, the
+thermal 256^2 x 50 and vpipes 1024^2 x 200 workloads (both variants, error
+against the FP64 oracle), the 96-map batch and the served Poisson stream.
 """
 import argparse
 import json
@@ -279,8 +283,53 @@ def run_headline(args):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n)
     if not args.no_extras:
+        out["storage_variants"] = storage_variants(args)
+        # rates only (device-generated inputs); their bit-exactness against
+        # the oracle at these sizes is the GPU tests'
+        out["config1_thermal256"] = run_thermal256(args, cpu=False, big_grid=False, check=False)
+        out["config2_vpipes1024"] = run_vpipes1024(args, cpu=False, check=False)["results"]
         out["batch"] = batch_summary(args)
         out["served"] = served_summary(args)
+    return out
+
+
+def storage_variants(args):
+    """The headline's fused kernel with BF16 and FP16 state (FP32
+    arithmetic): per-launch time at the tuned shape (a 60-iteration session
+    of the same input probes the shapes first), HIP events around 50
+    consecutive launches on the library's stream after 250 iterations."""
+    torch, hm, ctx = _gpu()
+    n = args.n
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    z = device_terrain(torch, hm, ctx, n, n, 2)
+    out = {}
+    for storage in ("bf16", "f16"):
+        pre = hm.Session(z, storage=STORAGE[storage], **WORKLOAD)
+        pre.vpipes(60)
+        pre.close()
+        sess = hm.Session(z, storage=STORAGE[storage], **WORKLOAD)
+        # past the water's spreading from rest, as the FP32 launches are
+        # timed after the headline's steps
+        sess.vpipes(250)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        m = 50
+        e0.record(stream)
+        sess.vpipes(m)
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / m
+        chunk_rows, prefetch, _ = sess.tuning()
+        sess.close()
+        alg = ALG_BYTES[storage] * n * n
+        out[storage] = {"avg_launch_us": round(us, 2),
+                        "kernel_gcell_iter_s": round(n * n / (us * 1e-6) / 1e9, 2),
+                        "alg_bytes_per_launch": alg,
+                        "achieved_gbs": round(alg / (us * 1e-6) / 1e9, 1),
+                        "frac": round(alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                        "launch_shape": {"chunk_rows": chunk_rows, "prefetch": prefetch}}
+    del z
+    torch.cuda.empty_cache()
     return out
 
 
@@ -548,16 +597,44 @@ def run_precision(args):
 # ------------------------------------------------ configs 1 and 2 (drop-in)
 
 
-def run_thermal256(args):
-    """hmap::gpu::thermal drop-in on 256^2 fBm (seed 1), talus 2/256, 50
-    iterations: host-pointer call (transfers included) and device-resident,
-    against the CPU restatement (1 thread and all threads) and the FP64 oracle."""
+def thermal256_checks(z0, z):
+    """The 256^2 thermal result against the FP32 oracle (bit for bit) and
+    the FP64 oracle (error, interior sum drift)."""
     import numpy as np
 
     from oracle import pyoracle as O
 
+    o64 = O.thermal(z0.astype(np.float64), 2.0 / 256, 50, np.float64)
+
+    def interior_sum(a):
+        return float(a[1:-1, 1:-1].astype(np.float64).sum())
+
+    s0 = interior_sum(z0)
+    return {"bitexact_vs_fp32_oracle": bool(np.array_equal(z, O.thermal(z0, 2.0 / 256, 50))),
+            "max_abs_err_rel_range_vs_fp64": float(np.abs(z - o64).max() / (o64.max() - o64.min())),
+            # relative change of the interior sum over the 50 steps (the
+            # exchange with border cells moves height across the interior's
+            # edge), for the GPU result and the FP64 oracle
+            "interior_sum_drift": (interior_sum(z) - s0) / s0,
+            "interior_sum_drift_fp64": (interior_sum(o64) - s0) / s0}
+
+
+def run_thermal256(args, cpu=True, big_grid=True, check=True):
+    """hmap::gpu::thermal drop-in on 256^2 fBm (seed 1), talus 2/256, 50
+    iterations: host-pointer call (transfers included) and device-resident,
+    against the CPU restatement (1 thread and all threads; cpu) and the FP64
+    oracle (check), and the same call at 4096^2 device-resident (big_grid).
+    Without check the input is generated on the device and only the rates
+    are reported (the oracle is not used)."""
+    import numpy as np
+
     torch, hm, ctx = _gpu()
-    z0 = O.fbm_terrain(256, 256, 1)
+    if check:
+        from oracle import pyoracle as O
+
+        z0 = O.fbm_terrain(256, 256, 1)
+    else:
+        z0 = device_terrain(torch, hm, ctx, 256, 256, 1).cpu().numpy()
     z = z0.copy()
     hm.thermal(z, 2.0 / 256, 50)
     reps = 20
@@ -574,48 +651,42 @@ def run_thermal256(args):
         hm._check(hm.lib().hmg_thermal_dev(ctx.handle, zd.data_ptr(), 256, 256, None, 2.0 / 256,
                                            50, None))
     dev_s = (time.perf_counter() - t0) / reps
-    o64 = O.thermal(z0.astype(np.float64), 2.0 / 256, 50, np.float64)
-    # the same call at 4096^2, device-resident (the streaming kernel, one
-    # step per launch): events around the call, best of 3
-    big = 4096
-    zb = device_terrain(torch, hm, ctx, big, big, 1)
-    st = torch.cuda.ExternalStream(ctx.stream)
-    big_ms = None
-    for _ in range(3):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        hm._check(hm.lib().hmg_thermal_dev(ctx.handle, zb.data_ptr(), big, big, None, 2.0 / big,
-                                           50, None))
-        e1.record(st)
-        e1.synchronize()
-        ms = e0.elapsed_time(e1)
-        big_ms = ms if big_ms is None else min(big_ms, ms)
-    big_rate = big * big * 50 / (big_ms / 1e3) / 1e9
-    c1 = cpu_bench("thermal", 256, 256, 50, 5, 1)
-    cn = cpu_bench("thermal", 256, 256, 50, 5)
     cells = 256 * 256 * 50
+    out = {"metric": "Gcell-iter/s thermal 256^2 x 50 (drop-in)",
+           "value": round(cells / call_s / 1e9, 4), "unit": "Gcell-iter/s", "n_gpus": 1,
+           "device_resident_gcell_iter_s": round(cells / dev_s / 1e9, 4)}
+    if check:
+        out.update(thermal256_checks(z0, z))
+    if big_grid:
 
-    def interior_sum(a):
-        return float(a[1:-1, 1:-1].astype(np.float64).sum())
-
-    s0 = interior_sum(z0)
-    return {"metric": "Gcell-iter/s thermal 256^2 x 50 (drop-in)",
-            "value": round(cells / call_s / 1e9, 4), "unit": "Gcell-iter/s", "n_gpus": 1,
-            "device_resident_gcell_iter_s": round(cells / dev_s / 1e9, 4),
-            "bitexact_vs_fp32_oracle": bool(np.array_equal(z, O.thermal(z0, 2.0 / 256, 50))),
-            "max_abs_err_rel_range_vs_fp64": float(np.abs(z - o64).max() / (o64.max() - o64.min())),
-            # relative change of the interior sum over the 50 steps (the
-            # exchange with border cells moves height across the interior's
-            # edge), for the GPU result and the FP64 oracle
-            "interior_sum_drift": (interior_sum(z) - s0) / s0,
-            "interior_sum_drift_fp64": (interior_sum(o64) - s0) / s0,
-            # 4096^2 x 50 resident: 8 B per cell-step is the one-step-per-pass
-            # HBM minimum; the blocked kernel moves less (8 steps per pass)
-            "resident_4096_gcell_iter_s": round(big_rate, 2),
-            "resident_4096_ms": round(big_ms, 3),
-            "resident_4096_gbs_at_8B_per_cell_step": round(big_rate * 8, 1),
-            "cpu_1_thread_gcell_iter_s": c1["gcell_iter_s"],
-            "cpu_all_threads_gcell_iter_s": cn["gcell_iter_s"], "cpu_threads": cn["threads"]}
+        # the same call at 4096^2, device-resident (two Jacobi steps per
+        # streaming launch): events around the call, best of 3
+        big = 4096
+        zb = device_terrain(torch, hm, ctx, big, big, 1)
+        st = torch.cuda.ExternalStream(ctx.stream)
+        big_ms = None
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            hm._check(hm.lib().hmg_thermal_dev(ctx.handle, zb.data_ptr(), big, big, None,
+                                               2.0 / big, 50, None))
+            e1.record(st)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            big_ms = ms if big_ms is None else min(big_ms, ms)
+        big_rate = big * big * 50 / (big_ms / 1e3) / 1e9
+        # 8 B per cell-step is one read and one write of z per step; the
+        # two-step launches move half of it
+        out.update({"resident_4096_gcell_iter_s": round(big_rate, 2),
+                    "resident_4096_ms": round(big_ms, 3),
+                    "resident_4096_gbs_at_8B_per_cell_step": round(big_rate * 8, 1)})
+    if cpu:
+        c1 = cpu_bench("thermal", 256, 256, 50, 5, 1)
+        cn = cpu_bench("thermal", 256, 256, 50, 5)
+        out.update({"cpu_1_thread_gcell_iter_s": c1["gcell_iter_s"],
+                    "cpu_all_threads_gcell_iter_s": cn["gcell_iter_s"],
+                    "cpu_threads": cn["threads"]})
+    return out
 
 
 def run_flow(args):
@@ -762,18 +833,22 @@ def run_tiles(args):
             "detail": res}
 
 
-def run_vpipes1024(args):
+def run_vpipes1024(args, cpu=True, check=True):
     """hydraulic_vpipes at 1024^2 (seed 2), 200 iterations: (i) reference
     semantics (gravity 1, volume top-up) with plain and fused launches,
-    (ii) gravity 9.81 with 1e-3 rain per iteration; GPU vs CPU restatement,
-    error vs the FP64 oracle."""
+    (ii) gravity 9.81 with 1e-3 rain per iteration; GPU vs CPU restatement
+    (cpu), error vs the FP64 oracle (check).  Without check the input is
+    generated on the device and only the rates are reported."""
     import numpy as np
-
-    from oracle import pyoracle as O
 
     torch, hm, ctx = _gpu()
     n, iters = 1024, 200
-    z0 = O.fbm_terrain(n, n, 2)
+    if check:
+        from oracle import pyoracle as O
+
+        z0 = O.fbm_terrain(n, n, 2)
+    else:
+        z0 = device_terrain(torch, hm, ctx, n, n, 2).cpu().numpy()
     res = {}
     for name, extra in (("i_reference", {}),
                         ("ii_gravity_rain", dict(gravity=9.81, rain_per_iter=1e-3,
@@ -798,14 +873,20 @@ def run_vpipes1024(args):
             row[mode + "_resident_gcell_iter_s"] = round(n * n * iters / (time.perf_counter() - t0)
                                                          / 1e9, 3)
             s.close()
-        o64 = O.VpState(z0.astype(np.float64), O.vp_params(**kw), dtype=np.float64).run(iters)
-        row["max_abs_err_rel_range_vs_fp64"] = float(np.abs(z - o64.z).max()
-                                                     / (o64.z.max() - o64.z.min()))
+        if check:
+            # these parameters are chaotic: FP32 arithmetic itself (the GPU
+            # equals the FP32 oracle bit for bit) separates from FP64 by
+            # O(range) within tens of iterations
+            o64 = O.VpState(z0.astype(np.float64), O.vp_params(**kw), dtype=np.float64).run(iters)
+            row["max_abs_err_rel_range_vs_fp64"] = float(np.abs(z - o64.z).max()
+                                                         / (o64.z.max() - o64.z.min()))
         res[name] = row
-    c = cpu_bench("vpipes", n, n, 20, 3)
-    return {"metric": "Gcell-iter/s vpipes 1024^2 x 200", "unit": "Gcell-iter/s", "n_gpus": 1,
-            "value": res["i_reference"]["fused_resident_gcell_iter_s"], "results": res,
-            "cpu_all_threads_gcell_iter_s": c["gcell_iter_s"], "cpu_threads": c["threads"]}
+    out = {"metric": "Gcell-iter/s vpipes 1024^2 x 200", "unit": "Gcell-iter/s", "n_gpus": 1,
+           "value": res["i_reference"]["fused_resident_gcell_iter_s"], "results": res}
+    if cpu:
+        c = cpu_bench("vpipes", n, n, 20, 3)
+        out.update({"cpu_all_threads_gcell_iter_s": c["gcell_iter_s"], "cpu_threads": c["threads"]})
+    return out
 
 
 _PROBE = None
@@ -1001,7 +1082,8 @@ def main():
                              "ceiling"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="headline: skip the batch and served-mix measurements")
+                    help="headline: skip the BF16/FP16 kernel rates, the thermal 256^2 and "
+                         "vpipes 1024^2 workloads, the batch and the served mix")
     args = ap.parse_args()
     if args.storage is None:
         args.storage = "bf16" if args.workload == "serve" else "f32"
