@@ -405,10 +405,12 @@ def run_batch(args):
     st = STORAGE[args.storage]
 
     # warm-up and autotune (shape, then graph against persistent launch) on
-    # a throw-away copy: 15 shapes and 2 forms of 2 probe iterations each
+    # a throw-away copy: 20 ordinary iterations, then the fixed and the
+    # occupancy-fitted shapes (up to 24) and the 2 forms, 2 probe
+    # iterations each
     b = hm.Batch(storage=st, **WORKLOAD)
     for z, m in zip(zs, maps):
-        b.add(z, 40)
+        b.add(z, 100)
     tuned = b.autotune(probe=2)
     form = b.tuned_form()
     b.close()

@@ -2168,10 +2168,71 @@ int hmg_batch_set_tuning(hmg_batch *b, int chunk_rows, int prefetch)
   return HMG_OK;
 }
 
+} // extern "C"
+
+// Workgroups of the batched kernel variant (prefetch 0: rows loaded when
+// needed, 1: one row ahead, 2: gathers ahead) one CU holds at once, from the
+// kernel's register use (0 if the runtime cannot tell).
+template <class S>
+static int batch_blocks_per_cu_t(bool rain, bool ke1, int pf)
+{
+  const void *f = nullptr;
+#define HMG_BATCH_KERNEL(UR, PF, KE, GA) (const void *)k_vp_batch<S, UR, PF, KE, GA>
+  if (pf == 2)
+    f = rain ? (ke1 ? HMG_BATCH_KERNEL(false, true, true, true) : HMG_BATCH_KERNEL(false, true, false, true))
+             : (ke1 ? HMG_BATCH_KERNEL(true, true, true, true) : HMG_BATCH_KERNEL(true, true, false, true));
+  else if (pf == 1)
+    f = rain ? (ke1 ? HMG_BATCH_KERNEL(false, true, true, false) : HMG_BATCH_KERNEL(false, true, false, false))
+             : (ke1 ? HMG_BATCH_KERNEL(true, true, true, false) : HMG_BATCH_KERNEL(true, true, false, false));
+  else
+    f = rain ? (ke1 ? HMG_BATCH_KERNEL(false, false, true, false) : HMG_BATCH_KERNEL(false, false, false, false))
+             : (ke1 ? HMG_BATCH_KERNEL(true, false, true, false) : HMG_BATCH_KERNEL(true, false, false, false));
+#undef HMG_BATCH_KERNEL
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64 * WPB, 0) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+static int batch_blocks_per_cu(const hmg_batch *b, int pf)
+{
+  bool rain = false;
+  for (const hmg_session *s : b->maps) rain = rain || s->rain.ptr != nullptr;
+  const bool ke1 = b->maps[0]->k.kexp == 1.f;
+  if (b->storage == HMG_STORAGE_BF16) return batch_blocks_per_cu_t<StBF16>(rain, ke1, pf);
+  if (b->storage == HMG_STORAGE_F16) return batch_blocks_per_cu_t<StF16>(rain, ke1, pf);
+  return batch_blocks_per_cu_t<StF32>(rain, ke1, pf);
+}
+
+// Work items of one launch over every map of the batch at chunk length
+// `rows` (capped at each map's height, as batch_advance_t does).
+static long batch_items(const hmg_batch *b, int rows)
+{
+  long n = 0;
+  for (const hmg_session *s : b->maps)
+  {
+    const int cr = rows < s->ny ? rows : s->ny;
+    n += (long)((s->nx + STRIP_OUT - 1) / STRIP_OUT) * ((s->ny + cr - 1) / cr);
+  }
+  return n;
+}
+
+extern "C" {
+
 // Times each candidate launch shape on `probe` iterations of the live maps
 // (results do not depend on the shape, so the probe iterations are part of
 // the run) and keeps the fastest; the choice is cached per storage type and
-// size class for later batches.
+// size class for later batches.  The candidates are a fixed list of chunk
+// lengths for each variant plus, per variant, the shortest chunks whose
+// work items fill one, two or three rounds of the wavefront slots the
+// kernel's occupancy gives (a nearly empty last round leaves most of the
+// GPU idle for a whole item's time).  When every map has iterations to
+// spare, the probes start after 20 ordinary iterations in the default
+// shape: while the water spreads from rest the shapes rank differently from
+// the rest of a run.
 int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefetch_out)
 {
   clear_error();
@@ -2197,10 +2258,42 @@ int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefet
     }
   }
   if (probe < 1) probe = 1;
-  static const int cand_rows[] = {16, 32, 48, 64, 96};
+  static const int                 cand_rows[] = {16, 32, 48, 64, 96};
+  std::vector<std::pair<int, int>> cand; // (rows, prefetch)
+  for (int pf = 0; pf < 3; pf++)         // rows loaded when needed, one ahead, gathers ahead
+    for (int rows : cand_rows) cand.push_back({rows, pf});
+  const int cus = b->ctx->num_cus > 0 ? b->ctx->num_cus : 256;
+  for (int pf = 0; pf < 3; pf++)
+  {
+    const int per_cu = batch_blocks_per_cu(b, pf);
+    if (per_cu <= 0) continue;
+    const long slots = (long)per_cu * WPB * cus;
+    for (int rounds = 1; rounds <= 3; rounds++)
+    {
+      int rows = 8;
+      while (rows <= 128 && batch_items(b, rows) > rounds * slots) rows++;
+      if (rows > 128) continue;
+      const std::pair<int, int> c{rows, pf};
+      if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    }
+  }
   float            best = 1e30f;
   int              best_rows = b->chunk_rows, best_pf = b->prefetch;
   std::vector<int> step(b->maps.size());
+  {
+    // 20 ordinary iterations first when every map can spare them besides
+    // the probes (both forms included)
+    const int need = probe * ((int)cand.size() + 2) + 20;
+    int       min_left = 1 << 30;
+    for (int it : b->iters) min_left = it < min_left ? it : min_left;
+    if (min_left >= need)
+    {
+      for (size_t i = 0; i < step.size(); i++) step[i] = 20;
+      HMG_TRY(batch_advance(b->ctx, b->maps.data(), step.data(), (int)step.size(), HMG_BATCH_GRAPH,
+                            b->chunk_rows, b->prefetch, nullptr));
+      for (size_t i = 0; i < step.size(); i++) b->iters[i] -= 20;
+    }
+  }
   // runs `probe` iterations of every map in one form and shape; false when
   // a map has fewer iterations left (every map's count stays exact)
   auto probe_run = [&](int form, int rows, int pf, float &ms, bool &ran) -> int
@@ -2216,19 +2309,18 @@ int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefet
     ran = true;
     return HMG_OK;
   };
-  for (int pf = 0; pf < 3; pf++) // rows loaded when needed, one ahead, gathers ahead
-    for (int rows : cand_rows)
+  for (const auto &c : cand)
+  {
+    float ms = 0.f;
+    bool  ran = false;
+    HMG_TRY(probe_run(HMG_BATCH_STREAM, c.first, c.second, ms, ran));
+    if (ran && ms < best)
     {
-      float ms = 0.f;
-      bool  ran = false;
-      HMG_TRY(probe_run(HMG_BATCH_STREAM, rows, pf, ms, ran));
-      if (ran && ms < best)
-      {
-        best = ms;
-        best_rows = rows;
-        best_pf = pf;
-      }
+      best = ms;
+      best_rows = c.first;
+      best_pf = c.second;
     }
+  }
   // the form, with the chosen shape: the captured launch sequence against
   // the persistent launch
   int   best_form = HMG_BATCH_GRAPH;

@@ -503,7 +503,10 @@ int hmg_batch_set_tuning(hmg_batch *b, int chunk_rows, int prefetch);
  * the live maps (they count toward each map's iterations; results do not
  * depend on the shape), then the form (graph or persistent, `probe`
  * iterations each with that shape); caches the choice per storage type and
- * batch size. */
+ * batch size.  Candidates: chunk lengths 16-96 for each variant, plus the
+ * lengths whose work items fill one, two or three rounds of the variant's
+ * wavefront slots.  When every map has 20 iterations to spare beyond the
+ * probes, 20 ordinary iterations run first. */
 int hmg_batch_autotune(hmg_batch *b, int probe, int *chunk_rows_out, int *prefetch_out);
 /* Forms of hmg_batch_run. */
 #define HMG_BATCH_STREAM 0     /* one launch per iteration on the stream */
