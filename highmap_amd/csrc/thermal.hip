@@ -17,6 +17,9 @@
 //                    once for the pair of cells it joins; the last step of a
 //                    thermal() call also applies extrapolate_borders.
 //                    8 B/cell (z read + write).
+//  k_thermal_stream2 two Jacobi steps per launch with a uniform talus, the
+//                    second from the first's rows kept in registers (calls of
+//                    two or more steps from 2^18 cells; 4 B per cell-step).
 //  k_thermal_step    one Jacobi step, one cell per lane, 9-point stencil, for
 //                    a talus plane (12 B/cell) or thermal_with_bedrock.
 //  k_thermal_blocked T Jacobi steps per launch on an LDS tile with a T-cell
@@ -24,6 +27,10 @@
 //                    where launches dominate.
 //  k_extrap_cols / k_extrap_rows  extrapolate_borders, columns then rows,
 //                    where it is not folded into the last step.
+//  k_thermal_auto_step  one thermal_auto_bedrock Jacobi step.
+//  k_thermal_post    the deposition map and the mask blend after a call.
+//  k_sd_pre / k_sd_post / k_sd_finish  sediment_deposition's element-wise
+//                    rounds around its thermal sub-iterations.
 #include <algorithm>
 #include <cfloat>
 #include <mutex>

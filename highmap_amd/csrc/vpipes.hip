@@ -25,15 +25,26 @@
 // k-1, adds into set k and clears set k+1.
 //
 // Kernels (storage type S in {f32, bf16, f16}):
-//   k_vp_fused     one launch per iteration.  A block owns a TW x TH tile,
-//                  stages z, water and fluxes of the tile plus a 2-cell halo
-//                  in LDS, then transport (1-cell halo), flow (1-cell halo),
-//                  water and erosion (tile), and writes z, D, F, Sp once.
-//                  HBM: 28 B read + 28 B written per cell (FP32 storage).
+//   k_vp_stream, k_vp_stream_ga
+//                  the fused form, one launch per iteration: each wavefront
+//                  streams a strip of 64 columns (60 written) down a chunk
+//                  of rows with a rolling register window, and writes z, D,
+//                  F and Sp once (HBM: 28 B read + 28 B written per cell in
+//                  FP32 storage); _ga issues each row's sediment gathers one
+//                  row step before it uses them.
+//   k_vp_batch, k_vp_batch_persist
+//                  the same work item over many maps: one launch per
+//                  iteration of every map, or one cooperative launch for
+//                  all iterations with a grid barrier between them.
+//   k_flow_stream  flow_simulation: the fused item with the flow and water
+//                  passes only.
 //   k_vp_prep, k_vp_flow, k_vp_water, k_vp_erosion, k_vp_sum
 //                  the plain form: one launch per pass, every intermediate
 //                  through HBM, as the reference structures its loop.
 //   k_vp_export    forms d, s, u, v as hydraulic_vpipes returns them.
+//   k_fill_state, k_nonfinite, k_absmax2, k_flow_init, k_dry_out
+//                  session set-up, the finite check, the backtrace bound
+//                  and flow_simulation's set-up and dry-out.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
