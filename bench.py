@@ -9,7 +9,7 @@ k_capacity 0.1, k_erode 0.05, k_depose 0.05, evaporation 0.01).  The default
 50 timed steps are one full 500-iteration job.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--storage f32|bf16|f16]
-    python bench.py --workload batch|serve|precision|thermal256|vpipes1024|flow|snow|tiles|copy
+    python bench.py --workload batch|serve|precision|thermal256|vpipes1024|flow|snow|tiles|copy|ceiling|noise
 
 Each run prints one JSON line.  For the headline, `value` is hydraulic
 cell-iterations per second (nx*ny*iterations / wall time of the timed steps,
@@ -18,7 +18,8 @@ thermal included in the time) with the state already resident in HBM;
 the library's stream; `access_pattern_ceiling` times the same kernel's
 memory operations alone (k_vp_pattern); `cpu_baseline` times the FP32 CPU
 restatement (oracle/cpu_bench) on a bounded sample of the same workload.
-Unless --no-extras: the kernel's rate with BF16 and FP16 state, the
+Unless --no-extras: the kernel's rate with BF16 and FP16 state, noise_fbm's
+rate (PERLIN, the input generator), the
 thermal 256^2 x 50 and vpipes 1024^2 x 200 workloads (both variants, error
 against the FP64 oracle), the 96-map batch and the served Poisson stream.
 """
@@ -284,6 +285,7 @@ def run_headline(args):
         out["cpu_baseline"] = cpu_baseline(n)
     if not args.no_extras:
         out["storage_variants"] = storage_variants(args)
+        out["noise_fbm_perlin"] = noise_rates(args)["1"]
         # rates only (device-generated inputs); their bit-exactness against
         # the oracle at these sizes is the GPU tests'
         out["config1_thermal256"] = run_thermal256(args, cpu=False, big_grid=False, check=False)
@@ -291,6 +293,40 @@ def run_headline(args):
         out["batch"] = batch_summary(args)
         out["served"] = served_summary(args)
     return out
+
+
+def noise_rates(args, types=(1,)):
+    """hmap::gpu::noise_fbm on the device at n x n (8 octaves, wavenumber
+    4, seed 2, no control or warping planes): HIP events around the call on
+    the library's stream, best of 3; Gcell/s per noise type.  VALU-bound
+    (hash sines and gradient sincos), reported as a rate, not against HBM."""
+    torch, hm, ctx = _gpu()
+    n = args.n
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    z = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    out = {}
+    for t in types:
+        best = None
+        for _ in range(3):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            hm._check(hm.lib().hmg_noise_fbm_dev(ctx.handle, z.data_ptr(), n, n, t, 4.0, 4.0, 2,
+                                                 8, 0.7, 0.5, 2.0, None, None, None, None, None))
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        out[str(t)] = {"ms": round(best, 4), "gcell_s": round(n * n / (best / 1e3) / 1e9, 2)}
+    del z
+    return out
+
+
+def run_noise(args):
+    """noise_fbm at n x n for every implemented hmap::NoiseType."""
+    r = noise_rates(args, types=(1, 2, 3, 4, 6, 7, 9, 10))
+    return {"metric": "Gcell/s noise_fbm %dx%d, 8 octaves" % (args.n, args.n), "unit": "Gcell/s",
+            "n_gpus": 1, "value": r["1"]["gcell_s"], "by_type": r}
 
 
 def storage_variants(args):
@@ -1081,7 +1117,7 @@ def main():
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batch", "batchforms", "serve", "precision",
                              "thermal256", "vpipes1024", "flow", "snow", "tiles", "copy",
-                             "ceiling"])
+                             "ceiling", "noise"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline: skip the BF16/FP16 kernel rates, the thermal 256^2 and "
@@ -1099,7 +1135,8 @@ def main():
     fn = {"headline": run_headline, "batch": run_batch, "batchforms": run_batch_forms,
           "precision": run_precision,
           "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "flow": run_flow,
-          "snow": run_snow, "tiles": run_tiles, "copy": run_copy, "ceiling": run_ceiling}
+          "snow": run_snow, "tiles": run_tiles, "copy": run_copy, "ceiling": run_ceiling,
+          "noise": run_noise}
     if args.workload == "serve":
         from highmap_amd import serve
 

@@ -35,12 +35,18 @@ __device__ __forceinline__ int scale_period(int period, float lacunarity)
   return period;
 }
 
+// grad22f at a lattice corner: (cos a, sin a), a = 2 pi hash12f.
+__device__ __forceinline__ void perlin_grad(float cx, float cy, float fseed, float &c, float &s)
+{
+  float angle = 6.2831853f * hash12f(cx, cy, fseed);
+  hm_sincos(angle, s, c);
+}
+
 // grad22f at a lattice corner dotted with the offset (dx, dy).
 __device__ __forceinline__ float perlin_corner(float cx, float cy, float fseed, float dx, float dy)
 {
-  float angle = 6.2831853f * hash12f(cx, cy, fseed);
   float s, c;
-  hm_sincos(angle, s, c);
+  perlin_grad(cx, cy, fseed, c, s);
   return c * dx + s * dy;
 }
 
@@ -68,6 +74,55 @@ __device__ __forceinline__ float base_perlin(float px, float py, float fseed, in
   float a = lerp_cl(n00, n10, ux);
   float b = lerp_cl(n01, n11, ux);
   return 1.42857f * lerp_cl(a, b, uy);
+}
+
+// base_perlin for a whole wavefront whose 64 lanes are 64 consecutive cells
+// of one row, without warping planes: all lanes share iy, and ix is
+// monotonic across the lanes (every step from the cell index to ix is), so
+// the lattice columns the wave needs are those between the first lane's and
+// the last valid lane's.  When they span at most 31 columns with |ix| below
+// 2^23 (integers exact, lo + k exact), lanes 0-31 form the gradients of
+// columns lo .. lo+31 on lattice row y0 and lanes 32-63 on row y1, once per
+// lattice point, and each lane fetches its four corners from those lanes:
+// the same hash12f and sincos of the same wrapped lattice coordinates as
+// base_perlin (lo + m is ix and lo + m + 1 is ix + 1 as floats), dotted and
+// blended in the same order, so the result is bit for bit base_perlin's.
+// Otherwise the wave takes base_perlin.  `last` is the last valid lane
+// (uniform); lanes past it compute but their results are not stored.
+__device__ __forceinline__ float base_perlin_wave(float px,
+                                                  float py,
+                                                  float fseed,
+                                                  int   per_x,
+                                                  int   per_y,
+                                                  int   lane,
+                                                  int   last)
+{
+  const float ix = floorf(px), iy = floorf(py);
+  const float ix_first = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ix)));
+  const float ix_last = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ix), last));
+  const float lo = fminf(ix_first, ix_last), hi = fmaxf(ix_first, ix_last);
+  // (a NaN at either end, or an infinity, fails the test)
+  if (!(ix_first == ix_first && ix_last == ix_last && hi - lo <= 30.f && fabsf(lo) < 0x1p23f &&
+        fabsf(hi) < 0x1p23f))
+    return base_perlin(px, py, fseed, per_x, per_y);
+  const float fx = fract_cl(px), fy = fract_cl(py);
+  const float y0 = wrap_lattice(iy, per_y), y1 = wrap_lattice(iy + 1.f, per_y);
+  float       gc, gs;
+  perlin_grad(wrap_lattice(lo + (float)(lane & 31), per_x), lane < 32 ? y0 : y1, fseed, gc, gs);
+  const int m = (int)(ix - lo); // 0 .. 30 on valid lanes
+  const int a = m & 63, b = (m + 1) & 63, c = (m + 32) & 63, d = (m + 33) & 63;
+  const float c00 = __shfl(gc, a, 64), s00 = __shfl(gs, a, 64);
+  const float c10 = __shfl(gc, b, 64), s10 = __shfl(gs, b, 64);
+  const float c01 = __shfl(gc, c, 64), s01 = __shfl(gs, c, 64);
+  const float c11 = __shfl(gc, d, 64), s11 = __shfl(gs, d, 64);
+  const float n00 = c00 * fx + s00 * fy;
+  const float n10 = c10 * (fx - 1.f) + s10 * fy;
+  const float n01 = c01 * fx + s01 * (fy - 1.f);
+  const float n11 = c11 * (fx - 1.f) + s11 * (fy - 1.f);
+  const float ux = fade5(fx), uy = fade5(fy);
+  const float ea = lerp_cl(n00, n10, ux);
+  const float eb = lerp_cl(n01, n11, ux);
+  return 1.42857f * lerp_cl(ea, eb, uy);
 }
 
 // Simplex noise on the skewed triangular lattice: the input is halved, the
@@ -179,12 +234,19 @@ __device__ __forceinline__ float max_smooth(float a, float b, float k)
   return fmaxf(a, b) + h * h * h * k / 6.f;
 }
 
+// The three Perlin types from one Perlin value.
+template <int TYPE>
+__device__ __forceinline__ float perlin_type(float v)
+{
+  if constexpr (TYPE == 2) return 2.f * fabsf(v) - 1.f;
+  if constexpr (TYPE == 3) return max_smooth(v, 0.f, 0.5f);
+  return v;
+}
+
 template <int TYPE>
 __device__ __forceinline__ float base_noise(float px, float py, float fseed, int per_x, int per_y)
 {
-  if constexpr (TYPE == 1) return base_perlin(px, py, fseed, per_x, per_y);
-  if constexpr (TYPE == 2) return 2.f * fabsf(base_perlin(px, py, fseed, per_x, per_y)) - 1.f;
-  if constexpr (TYPE == 3) return max_smooth(base_perlin(px, py, fseed, per_x, per_y), 0.f, 0.5f);
+  if constexpr (TYPE >= 1 && TYPE <= 3) return perlin_type<TYPE>(base_perlin(px, py, fseed, per_x, per_y));
   if constexpr (TYPE == 4) return base_simplex2(px, py, fseed);
   if constexpr (TYPE == 6) return base_value<true>(px, py, fseed, per_x, per_y);
   if constexpr (TYPE == 7) return base_value_cubic(px, py, fseed, per_x, per_y);
@@ -214,7 +276,12 @@ __device__ __forceinline__ void noise_xy(int i, int j, const NoiseArgs &a, float
   y = a.ky * (y * (a.b3 - a.b2) + a.b2) + a.ky * dy;
 }
 
-template <int TYPE>
+// WAVE: a Perlin type without warping planes, the lattice gradients shared
+// across each wavefront (base_perlin_wave).  Blocks of 64 x 4: a wavefront
+// is 64 consecutive cells of one row.  With WAVE, lanes past the grid's
+// last column stay live (the shared gradients need every lane) and store
+// nothing.
+template <int TYPE, bool WAVE = false>
 __global__ void __launch_bounds__(256) k_noise_fbm(float *__restrict__ out,
                                                    const float *__restrict__ ctrl,
                                                    const float *__restrict__ nxp,
@@ -223,18 +290,29 @@ __global__ void __launch_bounds__(256) k_noise_fbm(float *__restrict__ out,
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.nx || j >= a.ny) return;
-  size_t idx = (size_t)j * a.nx + i;
-  float  ct = ctrl ? ctrl[idx] : 1.f;
-  float  w = (1.f - ct) + ct * a.weight;
-  float  x, y;
-  noise_xy(i, j, a, nxp ? nxp[idx] : 0.f, nyp ? nyp[idx] : 0.f, x, y);
+  if (j >= a.ny) return; // the whole wavefront
+  if (!WAVE && i >= a.nx) return;
+  const bool   valid = i < a.nx;
+  const int    lane = threadIdx.x & 63;
+  const int    last = min(63, a.nx - 1 - (int)(blockIdx.x * blockDim.x)); // uniform
+  const size_t idx = (size_t)j * a.nx + i;
+  float        ct = (ctrl && valid) ? ctrl[idx] : 1.f;
+  float        w = (1.f - ct) + ct * a.weight;
+  float        x, y;
+  if (WAVE)
+    noise_xy(i, j, a, 0.f, 0.f, x, y);
+  else
+    noise_xy(i, j, a, nxp ? nxp[idx] : 0.f, nyp ? nyp[idx] : 0.f, x, y);
 
   float n = 0.f, nf = 1.f, na = 0.6f;
   int   px = a.per_x, py = a.per_y;
   for (int k = 0; k < a.octaves; k++)
   {
-    float v = base_noise<TYPE>(x * nf, y * nf, a.fseed, px, py);
+    float v;
+    if constexpr (WAVE)
+      v = perlin_type<TYPE>(base_perlin_wave(x * nf, y * nf, a.fseed, px, py, lane, last));
+    else
+      v = base_noise<TYPE>(x * nf, y * nf, a.fseed, px, py);
     n += v * na;
     na *= (1.f - w) + w * fminf(v + 1.f, 2.f) * 0.5f;
     na *= a.persistence;
@@ -242,7 +320,7 @@ __global__ void __launch_bounds__(256) k_noise_fbm(float *__restrict__ out,
     px = scale_period(px, a.lacunarity);
     py = scale_period(py, a.lacunarity);
   }
-  out[idx] = n;
+  if (valid) out[idx] = n;
 }
 
 template <int TYPE>
@@ -315,11 +393,19 @@ int launch_noise_fbm(float       *out,
   dim3 blk(64, 4);
   dim3 grd((nx + 63) / 64, (ny + 3) / 4);
 #define HMG_FBM(T) k_noise_fbm<T><<<grd, blk, 0, st>>>(out, ctrl, noise_x, noise_y, a)
+  // the Perlin types without warping planes share lattice gradients
+  // across each wavefront
+  const bool wave = !noise_x && !noise_y;
+#define HMG_FBM_PERLIN(T)                                                              \
+  if (wave)                                                                            \
+    k_noise_fbm<T, true><<<grd, blk, 0, st>>>(out, ctrl, nullptr, nullptr, a);         \
+  else                                                                                 \
+    HMG_FBM(T)
   switch (noise_type)
   {
-  case 1: HMG_FBM(1); break;
-  case 2: HMG_FBM(2); break;
-  case 3: HMG_FBM(3); break;
+  case 1: HMG_FBM_PERLIN(1); break;
+  case 2: HMG_FBM_PERLIN(2); break;
+  case 3: HMG_FBM_PERLIN(3); break;
   case 4: HMG_FBM(4); break;
   case 6: HMG_FBM(6); break;
   case 7: HMG_FBM(7); break;
@@ -327,6 +413,7 @@ int launch_noise_fbm(float       *out,
   case 10: HMG_FBM(10); break;
   default: HMG_FBM(0); break; // no kernel branch: zeros
   }
+#undef HMG_FBM_PERLIN
 #undef HMG_FBM
   return check_launch("noise_fbm");
 }

@@ -50,3 +50,33 @@ def test_noise_fbm_large_simplex_and_worley(hm, orc):
         g = hm.noise_fbm(t, (1024, 512), (16.0, 8.0), 123456789)
         o = orc.noise_fbm(1024, 512, (16.0, 8.0), 123456789, noise_type=t)
         assert np.array_equal(g, o), t
+
+
+# The Perlin types without warping planes share each wavefront's lattice
+# gradients (64 cells of one row, at most 31 lattice columns, |ix| < 2^23);
+# other waves compute per cell.  Both must give the oracle's bits: partial
+# wavefronts at the right edge, periods, a reversed bounding box and a
+# negative wavenumber (lattice index decreasing along the row), frequencies
+# where only some octaves fit the shared form, huge coordinates, and a
+# negative lacunarity.
+@pytest.mark.parametrize("noise_type", [1, 2, 3])
+@pytest.mark.parametrize("case", [
+    dict(shape=(130, 77), kw=(4.0, 6.0), octaves=5, bbox=(-0.5, 1.5, 0.25, 2.0), period=(4, 6)),
+    dict(shape=(193, 5), kw=(5.0, 3.0), octaves=8, bbox=(1.0, 0.0, 1.0, 0.0), period=(0, 0)),
+    dict(shape=(64, 3), kw=(-7.0, 2.0), octaves=6, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
+    dict(shape=(65, 1), kw=(3.0, 3.0), octaves=3, bbox=(0.0, 1.0, 0.0, 1.0), period=(3, 0)),
+    dict(shape=(256, 33), kw=(96.0, 8.0), octaves=4, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
+    dict(shape=(300, 9), kw=(2.0, 2.0), octaves=4, bbox=(4194290.0, 4194310.0, 0.0, 1.0),
+         period=(0, 0)),
+    dict(shape=(150, 20), kw=(4.0, 4.0), octaves=4, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0),
+         lacunarity=-2.0),
+])
+def test_noise_fbm_perlin_shared_gradients(hm, orc, noise_type, case):
+    nx, ny = case["shape"]
+    rng = np.random.default_rng(nx * ny)
+    ctrl = rng.random((ny, nx), dtype=np.float32)
+    kw = dict(octaves=case["octaves"], weight=0.4, persistence=0.5,
+              lacunarity=case.get("lacunarity", 2.0), bbox=case["bbox"], period=case["period"])
+    g = hm.noise_fbm(noise_type, (nx, ny), case["kw"], 9, ctrl_param=ctrl, **kw)
+    o = orc.noise_fbm(nx, ny, case["kw"], 9, ctrl=ctrl, noise_type=noise_type, **kw)
+    assert np.array_equal(g, o, equal_nan=True), (case, np.nanmax(np.abs(g - o)))
