@@ -333,7 +333,8 @@ def storage_variants(args):
     """The headline's fused kernel with BF16 and FP16 state (FP32
     arithmetic): per-launch time at the tuned shape (a 60-iteration session
     of the same input probes the shapes first), HIP events around 50
-    consecutive launches on the library's stream after 250 iterations."""
+    consecutive launches on the library's stream after 300 iterations
+    (with the headline's thermal steps)."""
     torch, hm, ctx = _gpu()
     n = args.n
     stream = torch.cuda.ExternalStream(ctx.stream)
@@ -344,9 +345,12 @@ def storage_variants(args):
         pre.vpipes(60)
         pre.close()
         sess = hm.Session(z, storage=STORAGE[storage], **WORKLOAD)
-        # past the water's spreading from rest, as the FP32 launches are
-        # timed after the headline's steps
-        sess.vpipes(250)
+        # past the water's spreading from rest (wide sediment backtraces make
+        # the first ~150 iterations slower), on the headline's state: 10
+        # hydraulic iterations then a thermal step, 30 times
+        for _ in range(30):
+            sess.vpipes(10)
+            sess.thermal(1.0 / n, 1)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         m = 50
