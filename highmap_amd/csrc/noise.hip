@@ -11,8 +11,11 @@
 // VALUE_LINEAR 9, WORLEY 10; the kernels write 0 for the other types and so
 // does this file.
 //
-// The kernels are VALU-bound (hash sines and sincos per lattice corner per
-// octave), not memory-bound: one 4-byte store per cell.  sin/cos are the
+// The kernels are VALU-bound (hash sines and sincos per lattice point per
+// octave), not memory-bound: one 4-byte store per cell.  Without warping
+// planes, each wavefront (64 cells of one row) forms every lattice value it
+// needs once, one or two per lane, and the cells gather them by cross-lane
+// permute (the *_wave forms), for every type but SIMPLEX2.  sin/cos are the
 // project-written routines of hmg_cell.hpp, so each field matches the FP32
 // restatement bit for bit.  One kernel instance per type (template), so the
 // per-cell work carries no type dispatch.
@@ -76,19 +79,33 @@ __device__ __forceinline__ float base_perlin(float px, float py, float fseed, in
   return 1.42857f * lerp_cl(a, b, uy);
 }
 
+// The lattice columns a wavefront needs, for the wave forms below: all lanes
+// share iy, and ix is monotonic across the lanes (every step from the cell
+// index to ix is), so the columns are those between the first lane's ix and
+// the last valid lane's (`last`, uniform).  True, with lo the smaller end,
+// when they span at most `span` columns and |ix| < 2^23 at both ends (lattice
+// integers within a few columns of them are then exact floats); false for a
+// NaN or an infinity at either end.
+__device__ __forceinline__ bool wave_lattice_span(float ix, int last, float span, float &lo)
+{
+  const float ix_first = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ix)));
+  const float ix_last = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ix), last));
+  lo = fminf(ix_first, ix_last);
+  const float hi = fmaxf(ix_first, ix_last);
+  return ix_first == ix_first && ix_last == ix_last && hi - lo <= span && fabsf(lo) < 0x1p23f &&
+         fabsf(hi) < 0x1p23f;
+}
+
 // base_perlin for a whole wavefront whose 64 lanes are 64 consecutive cells
-// of one row, without warping planes: all lanes share iy, and ix is
-// monotonic across the lanes (every step from the cell index to ix is), so
-// the lattice columns the wave needs are those between the first lane's and
-// the last valid lane's.  When they span at most 31 columns with |ix| below
-// 2^23 (integers exact, lo + k exact), lanes 0-31 form the gradients of
-// columns lo .. lo+31 on lattice row y0 and lanes 32-63 on row y1, once per
-// lattice point, and each lane fetches its four corners from those lanes:
-// the same hash12f and sincos of the same wrapped lattice coordinates as
-// base_perlin (lo + m is ix and lo + m + 1 is ix + 1 as floats), dotted and
-// blended in the same order, so the result is bit for bit base_perlin's.
-// Otherwise the wave takes base_perlin.  `last` is the last valid lane
-// (uniform); lanes past it compute but their results are not stored.
+// of one row, without warping planes.  When the wave's lattice columns span
+// at most 31 (wave_lattice_span), lanes 0-31 form the gradients of columns
+// lo .. lo+31 on lattice row y0 and lanes 32-63 on row y1, once per lattice
+// point, and each lane fetches its four corners from those lanes: the same
+// hash12f and sincos of the same wrapped lattice coordinates as base_perlin
+// (lo + m is ix and lo + m + 1 is ix + 1 as floats), dotted and blended in
+// the same order, so the result is bit for bit base_perlin's.  Otherwise
+// the wave takes base_perlin.  `last` is the last valid lane (uniform);
+// lanes past it compute but their results are not stored.
 __device__ __forceinline__ float base_perlin_wave(float px,
                                                   float py,
                                                   float fseed,
@@ -98,13 +115,8 @@ __device__ __forceinline__ float base_perlin_wave(float px,
                                                   int   last)
 {
   const float ix = floorf(px), iy = floorf(py);
-  const float ix_first = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ix)));
-  const float ix_last = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ix), last));
-  const float lo = fminf(ix_first, ix_last), hi = fmaxf(ix_first, ix_last);
-  // (a NaN at either end, or an infinity, fails the test)
-  if (!(ix_first == ix_first && ix_last == ix_last && hi - lo <= 30.f && fabsf(lo) < 0x1p23f &&
-        fabsf(hi) < 0x1p23f))
-    return base_perlin(px, py, fseed, per_x, per_y);
+  float       lo;
+  if (!wave_lattice_span(ix, last, 30.f, lo)) return base_perlin(px, py, fseed, per_x, per_y);
   const float fx = fract_cl(px), fy = fract_cl(py);
   const float y0 = wrap_lattice(iy, per_y), y1 = wrap_lattice(iy + 1.f, per_y);
   float       gc, gs;
@@ -167,6 +179,33 @@ __device__ __forceinline__ float base_value(float px, float py, float fseed, int
   return 2.f * lerp_cl(lerp_cl(v00, v10, ux), lerp_cl(v01, v11, ux), uy) - 1.f;
 }
 
+// base_value for a wavefront of one row (as base_perlin_wave): lanes 0-31
+// hash columns lo .. lo+31 of lattice row y0, lanes 32-63 of row y1, and
+// each lane fetches its four corners; the same hashes of the same wrapped
+// coordinates, blended in the same order.
+template <bool FADE>
+__device__ __forceinline__ float base_value_wave(float px,
+                                                 float py,
+                                                 float fseed,
+                                                 int   per_x,
+                                                 int   per_y,
+                                                 int   lane,
+                                                 int   last)
+{
+  const float ix = floorf(px), iy = floorf(py);
+  float       lo;
+  if (!wave_lattice_span(ix, last, 30.f, lo)) return base_value<FADE>(px, py, fseed, per_x, per_y);
+  const float fx = fract_cl(px), fy = fract_cl(py);
+  const float y0 = wrap_lattice(iy, per_y), y1 = wrap_lattice(iy + 1.f, per_y);
+  const float h = hash12f(wrap_lattice(lo + (float)(lane & 31), per_x), lane < 32 ? y0 : y1, fseed);
+  const int   m = (int)(ix - lo); // 0 .. 30 on valid lanes
+  const float v00 = __shfl(h, m & 63, 64), v10 = __shfl(h, (m + 1) & 63, 64);
+  const float v01 = __shfl(h, (m + 32) & 63, 64), v11 = __shfl(h, (m + 33) & 63, 64);
+  const float ux = FADE ? fx * fx * (3.f - 2.f * fx) : fx;
+  const float uy = FADE ? fy * fy * (3.f - 2.f * fy) : fy;
+  return 2.f * lerp_cl(lerp_cl(v00, v10, ux), lerp_cl(v01, v11, ux), uy) - 1.f;
+}
+
 // Cubic through four samples at t in [0, 1] between p1 and p2, evaluated
 // left to right as a t^3 + b t^2 + c t + d (cubic_interp,
 // _common_math.cl:30-37).
@@ -192,6 +231,39 @@ __device__ __forceinline__ float base_value_cubic(float px, float py, float fsee
     float       v[4];
 #pragma unroll
     for (int dx = 0; dx < 4; dx++) v[dx] = hash12f(wrap_lattice(ix + (float)(dx - 1), per_x), yy, fseed);
+    row[dy] = cubic4(v[0], v[1], v[2], v[3], fx);
+  }
+  return 1.43f * (cubic4(row[0], row[1], row[2], row[3], fy) - 0.5f);
+}
+
+// base_value_cubic for a wavefront of one row (as base_perlin_wave): when
+// the wave's lattice columns span at most 13, lane 16 r + c hashes column
+// lo - 1 + c of lattice row iy - 1 + r (r 0-3, c 0-15), and each lane
+// fetches its 4 x 4 block (columns ix - 1 .. ix + 2 are lanes m .. m + 3 of
+// each row, m = ix - lo); the same hashes of the same wrapped coordinates,
+// interpolated in the same order.
+__device__ __forceinline__ float base_value_cubic_wave(float px,
+                                                       float py,
+                                                       float fseed,
+                                                       int   per_x,
+                                                       int   per_y,
+                                                       int   lane,
+                                                       int   last)
+{
+  const float ix = floorf(px), iy = floorf(py);
+  float       lo;
+  if (!wave_lattice_span(ix, last, 12.f, lo)) return base_value_cubic(px, py, fseed, per_x, per_y);
+  const float fx = fract_cl(px), fy = fract_cl(py);
+  const float h = hash12f(wrap_lattice(lo + (float)((lane & 15) - 1), per_x),
+                          wrap_lattice(iy + (float)((lane >> 4) - 1), per_y), fseed);
+  const int   m = (int)(ix - lo); // 0 .. 12 on valid lanes
+  float       row[4];
+#pragma unroll
+  for (int dy = 0; dy < 4; dy++)
+  {
+    float v[4];
+#pragma unroll
+    for (int dx = 0; dx < 4; dx++) v[dx] = __shfl(h, (16 * dy + m + dx) & 63, 64);
     row[dy] = cubic4(v[0], v[1], v[2], v[3], fx);
   }
   return 1.43f * (cubic4(row[0], row[1], row[2], row[3], fy) - 0.5f);
@@ -227,6 +299,42 @@ __device__ __forceinline__ float base_worley(float px, float py, float fseed, in
   return 1.66f * dmin - 1.f;
 }
 
+// base_worley for a wavefront of one row (as base_perlin_wave): when the
+// wave's lattice columns span at most 19, lane 21 r + c forms the feature
+// point of column lo - 1 + c in lattice row iy - 1 + r (r 0-2, c 0-20; lane
+// 63 forms an unused one), and each lane fetches its 3 x 3 cells in the
+// reference's scan order; the same hash22f of the same wrapped coordinates.
+__device__ __forceinline__ float base_worley_wave(float px,
+                                                  float py,
+                                                  float fseed,
+                                                  int   per_x,
+                                                  int   per_y,
+                                                  int   lane,
+                                                  int   last)
+{
+  const float ix = floorf(px), iy = floorf(py);
+  float       lo;
+  if (!wave_lattice_span(ix, last, 18.f, lo)) return base_worley(px, py, fseed, per_x, per_y);
+  const float fx = fract_cl(px), fy = fract_cl(py);
+  const int   r = lane >= 42 ? 2 : lane >= 21 ? 1 : 0, c = lane - 21 * r;
+  float       hx, hy;
+  hash22f(wrap_lattice(lo + (float)(c - 1), per_x), wrap_lattice(iy + (float)(r - 1), per_y),
+          fseed, hx, hy);
+  const int m = (int)(ix - lo); // 0 .. 18 on valid lanes
+  float     dmin = FLT_MAX;
+#pragma unroll
+  for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+    {
+      const int   k = (21 * (dy + 1) + m + dx + 1) & 63;
+      const float gx = __shfl(hx, k, 64), gy = __shfl(hy, k, 64);
+      const float ex = fx - ((float)dx + gx), ey = fy - ((float)dy + gy);
+      dmin = fminf(dmin, ex * ex + ey * ey);
+    }
+  return 1.66f * dmin - 1.f;
+}
+
 // max_smooth(a, b, k) (_common_math.cl:52-56), pow_int(h, 3) as ((1 h) h) h.
 __device__ __forceinline__ float max_smooth(float a, float b, float k)
 {
@@ -255,6 +363,26 @@ __device__ __forceinline__ float base_noise(float px, float py, float fseed, int
   return 0.f;
 }
 
+// The wavefront forms (64 cells of one row, no warping planes): every type
+// but SIMPLEX2, whose lattice row varies along the row.
+template <int TYPE>
+__device__ __forceinline__ float base_noise_wave(float px,
+                                                 float py,
+                                                 float fseed,
+                                                 int   per_x,
+                                                 int   per_y,
+                                                 int   lane,
+                                                 int   last)
+{
+  if constexpr (TYPE >= 1 && TYPE <= 3)
+    return perlin_type<TYPE>(base_perlin_wave(px, py, fseed, per_x, per_y, lane, last));
+  if constexpr (TYPE == 6) return base_value_wave<true>(px, py, fseed, per_x, per_y, lane, last);
+  if constexpr (TYPE == 7) return base_value_cubic_wave(px, py, fseed, per_x, per_y, lane, last);
+  if constexpr (TYPE == 9) return base_value_wave<false>(px, py, fseed, per_x, per_y, lane, last);
+  if constexpr (TYPE == 10) return base_worley_wave(px, py, fseed, per_x, per_y, lane, last);
+  return base_noise<TYPE>(px, py, fseed, per_x, per_y);
+}
+
 struct NoiseArgs
 {
   int   nx, ny;
@@ -276,8 +404,8 @@ __device__ __forceinline__ void noise_xy(int i, int j, const NoiseArgs &a, float
   y = a.ky * (y * (a.b3 - a.b2) + a.b2) + a.ky * dy;
 }
 
-// WAVE: a Perlin type without warping planes, the lattice gradients shared
-// across each wavefront (base_perlin_wave).  Blocks of 64 x 4: a wavefront
+// WAVE: no warping planes, the lattice values shared across each
+// wavefront (base_noise_wave).  Blocks of 64 x 4: a wavefront
 // is 64 consecutive cells of one row.  With WAVE, lanes past the grid's
 // last column stay live (the shared gradients need every lane) and store
 // nothing.
@@ -310,7 +438,7 @@ __global__ void __launch_bounds__(256) k_noise_fbm(float *__restrict__ out,
   {
     float v;
     if constexpr (WAVE)
-      v = perlin_type<TYPE>(base_perlin_wave(x * nf, y * nf, a.fseed, px, py, lane, last));
+      v = base_noise_wave<TYPE>(x * nf, y * nf, a.fseed, px, py, lane, last);
     else
       v = base_noise<TYPE>(x * nf, y * nf, a.fseed, px, py);
     n += v * na;
@@ -393,27 +521,27 @@ int launch_noise_fbm(float       *out,
   dim3 blk(64, 4);
   dim3 grd((nx + 63) / 64, (ny + 3) / 4);
 #define HMG_FBM(T) k_noise_fbm<T><<<grd, blk, 0, st>>>(out, ctrl, noise_x, noise_y, a)
-  // the Perlin types without warping planes share lattice gradients
-  // across each wavefront
+  // without warping planes, every type but SIMPLEX2 shares its lattice
+  // values across each wavefront
   const bool wave = !noise_x && !noise_y;
-#define HMG_FBM_PERLIN(T)                                                              \
+#define HMG_FBM_WAVE(T)                                                                \
   if (wave)                                                                            \
     k_noise_fbm<T, true><<<grd, blk, 0, st>>>(out, ctrl, nullptr, nullptr, a);         \
   else                                                                                 \
     HMG_FBM(T)
   switch (noise_type)
   {
-  case 1: HMG_FBM_PERLIN(1); break;
-  case 2: HMG_FBM_PERLIN(2); break;
-  case 3: HMG_FBM_PERLIN(3); break;
+  case 1: HMG_FBM_WAVE(1); break;
+  case 2: HMG_FBM_WAVE(2); break;
+  case 3: HMG_FBM_WAVE(3); break;
   case 4: HMG_FBM(4); break;
-  case 6: HMG_FBM(6); break;
-  case 7: HMG_FBM(7); break;
-  case 9: HMG_FBM(9); break;
-  case 10: HMG_FBM(10); break;
+  case 6: HMG_FBM_WAVE(6); break;
+  case 7: HMG_FBM_WAVE(7); break;
+  case 9: HMG_FBM_WAVE(9); break;
+  case 10: HMG_FBM_WAVE(10); break;
   default: HMG_FBM(0); break; // no kernel branch: zeros
   }
-#undef HMG_FBM_PERLIN
+#undef HMG_FBM_WAVE
 #undef HMG_FBM
   return check_launch("noise_fbm");
 }

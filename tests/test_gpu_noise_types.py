@@ -52,14 +52,15 @@ def test_noise_fbm_large_simplex_and_worley(hm, orc):
         assert np.array_equal(g, o), t
 
 
-# The Perlin types without warping planes share each wavefront's lattice
-# gradients (64 cells of one row, at most 31 lattice columns, |ix| < 2^23);
-# other waves compute per cell.  Both must give the oracle's bits: partial
+# Without warping planes every type but SIMPLEX2 shares each wavefront's
+# lattice values (64 cells of one row; Perlin and VALUE / VALUE_LINEAR at
+# most 31 lattice columns, VALUE_CUBIC 13, WORLEY 19; |ix| < 2^23); other
+# waves compute per cell.  Both must give the oracle's bits: partial
 # wavefronts at the right edge, periods, a reversed bounding box and a
 # negative wavenumber (lattice index decreasing along the row), frequencies
-# where only some octaves fit the shared form, huge coordinates, and a
-# negative lacunarity.
-@pytest.mark.parametrize("noise_type", [1, 2, 3])
+# where only some octaves fit the shared form, spans on either side of the
+# VALUE_CUBIC and WORLEY limits, huge coordinates, and a negative lacunarity.
+@pytest.mark.parametrize("noise_type", [1, 2, 3, 6, 7, 9, 10])
 @pytest.mark.parametrize("case", [
     dict(shape=(130, 77), kw=(4.0, 6.0), octaves=5, bbox=(-0.5, 1.5, 0.25, 2.0), period=(4, 6)),
     dict(shape=(193, 5), kw=(5.0, 3.0), octaves=8, bbox=(1.0, 0.0, 1.0, 0.0), period=(0, 0)),
@@ -70,8 +71,13 @@ def test_noise_fbm_large_simplex_and_worley(hm, orc):
          period=(0, 0)),
     dict(shape=(150, 20), kw=(4.0, 4.0), octaves=4, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0),
          lacunarity=-2.0),
+    dict(shape=(64, 4), kw=(13.0, 2.0), octaves=2, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
+    dict(shape=(64, 4), kw=(14.0, 2.0), octaves=1, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
+    dict(shape=(128, 6), kw=(19.0, 3.0), octaves=2, bbox=(0.25, 1.25, 0.0, 1.0), period=(5, 3)),
+    dict(shape=(64, 4), kw=(19.0, 2.0), octaves=1, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
+    dict(shape=(64, 4), kw=(19.5, 2.0), octaves=1, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
 ])
-def test_noise_fbm_perlin_shared_gradients(hm, orc, noise_type, case):
+def test_noise_fbm_shared_lattice(hm, orc, noise_type, case):
     nx, ny = case["shape"]
     rng = np.random.default_rng(nx * ny)
     ctrl = rng.random((ny, nx), dtype=np.float32)
