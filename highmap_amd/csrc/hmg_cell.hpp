@@ -198,7 +198,10 @@ __device__ __forceinline__ float reduce_pio2(float x, int &q)
   float r = fmaf(-k, 0x1.921fb6p+0f, x);
   r = fmaf(-k, -0x1.777a5cp-25f, r);
   r = fmaf(-k, -0x1.ee59dap-50f, r);
-  q = ((int)k) & 3;
+  // q = k mod 4 for every k: floats of magnitude 2^25 and above are
+  // multiples of 4, so clamping there keeps the quadrant and keeps the
+  // conversion in range (a NaN k gives 0)
+  q = ((int)fminf(fmaxf(k, -0x1p25f), 0x1p25f)) & 3;
   return r;
 }
 

@@ -15,11 +15,12 @@
 // octave), not memory-bound: one 4-byte store per cell.  Without warping
 // planes, each wavefront (64 cells of one row) forms every lattice value it
 // needs once, one or two per lane, and the cells gather them by cross-lane
-// permute (the *_wave forms), for every type but SIMPLEX2.  sin/cos are the
+// permute (the *_wave forms).  sin/cos are the
 // project-written routines of hmg_cell.hpp, so each field matches the FP32
 // restatement bit for bit.  One kernel instance per type (template), so the
 // per-cell work carries no type dispatch.
 #include <cfloat>
+#include <climits>
 
 #include "hmg_internal.hpp"
 
@@ -32,9 +33,16 @@ __device__ __forceinline__ float wrap_lattice(float c, int period)
   return c;
 }
 
+// scale_period with a saturating conversion (NaN to 0, beyond the int range
+// to INT_MIN / INT_MAX), so that periods past 2^31 after many octaves are
+// defined.
 __device__ __forceinline__ int scale_period(int period, float lacunarity)
 {
-  if (period > 0) period = (int)((float)period * lacunarity + 0.5f);
+  if (period > 0)
+  {
+    const float p = (float)period * lacunarity + 0.5f;
+    period = p != p ? 0 : p >= 0x1p31f ? INT_MAX : p < -0x1p31f ? INT_MIN : (int)p;
+  }
   return period;
 }
 
@@ -159,6 +167,46 @@ __device__ __forceinline__ float base_simplex2(float px, float py, float fseed)
   {
     const float w = fmaxf(0.5f - (vx[k] * vx[k] + vy[k] * vy[k]), 0.f);
     n[k] = w * w * w * w * perlin_corner(lx[k], ly[k], fseed, vx[k], vy[k]);
+  }
+  return 100.f * (n[0] + n[1] + n[2]);
+}
+
+// base_simplex2 for a wavefront of one row (as base_perlin_wave): along the
+// row both skewed lattice indices are monotonic (x + skew and y + skew grow
+// or shrink with x), so the first and the last valid lane bound them.  When
+// they span at most 15 columns and 3 rows, lane 16 r + c forms the gradient
+// of lattice point (lo_x + c, lo_y + r), and each lane fetches those of its
+// three vertices (ix, iy), (ix + ox, iy + oy) and (ix + 1, iy + 1); the
+// same hash12f and sincos of the same lattice coordinates, dotted, weighted
+// and summed in the same order.
+__device__ __forceinline__ float base_simplex2_wave(float px, float py, float fseed, int lane,
+                                                    int last)
+{
+  constexpr float K1 = 0.366025403f, K2 = 0.211324865f;
+  const float     x = 0.5f * px, y = 0.5f * py;
+  const float     skew = x * K1 + y * K1;
+  const float     ix = floorf(x + skew), iy = floorf(y + skew);
+  float           lox, loy;
+  if (!wave_lattice_span(ix, last, 14.f, lox) || !wave_lattice_span(iy, last, 2.f, loy))
+    return base_simplex2(px, py, fseed);
+  float gc, gs;
+  perlin_grad(lox + (float)(lane & 15), loy + (float)(lane >> 4), fseed, gc, gs);
+  const float unskew = ix * K2 + iy * K2;
+  const float ax = (x - ix) + unskew, ay = (y - iy) + unskew;
+  const float m = ax < ay ? 0.f : 1.f; // second vertex: +x or +y
+  const float ox = m, oy = 1.f - m;
+  const float vx[3] = {ax, (ax - ox) + K2, (ax - 1.f) + 2.f * K2};
+  const float vy[3] = {ay, (ay - oy) + K2, (ay - 1.f) + 2.f * K2};
+  const int   b = 16 * (int)(iy - loy) + (int)(ix - lox); // 16 by + bx, bx <= 14, by <= 2
+  const int   mx = ax < ay ? 0 : 1;
+  const int   k3[3] = {b, b + 16 * (1 - mx) + mx, b + 17};
+  float       n[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+  {
+    const float w = fmaxf(0.5f - (vx[k] * vx[k] + vy[k] * vy[k]), 0.f);
+    const float c = __shfl(gc, k3[k] & 63, 64), sn = __shfl(gs, k3[k] & 63, 64);
+    n[k] = w * w * w * w * (c * vx[k] + sn * vy[k]);
   }
   return 100.f * (n[0] + n[1] + n[2]);
 }
@@ -363,8 +411,7 @@ __device__ __forceinline__ float base_noise(float px, float py, float fseed, int
   return 0.f;
 }
 
-// The wavefront forms (64 cells of one row, no warping planes): every type
-// but SIMPLEX2, whose lattice row varies along the row.
+// The wavefront forms (64 cells of one row, no warping planes).
 template <int TYPE>
 __device__ __forceinline__ float base_noise_wave(float px,
                                                  float py,
@@ -376,6 +423,7 @@ __device__ __forceinline__ float base_noise_wave(float px,
 {
   if constexpr (TYPE >= 1 && TYPE <= 3)
     return perlin_type<TYPE>(base_perlin_wave(px, py, fseed, per_x, per_y, lane, last));
+  if constexpr (TYPE == 4) return base_simplex2_wave(px, py, fseed, lane, last);
   if constexpr (TYPE == 6) return base_value_wave<true>(px, py, fseed, per_x, per_y, lane, last);
   if constexpr (TYPE == 7) return base_value_cubic_wave(px, py, fseed, per_x, per_y, lane, last);
   if constexpr (TYPE == 9) return base_value_wave<false>(px, py, fseed, per_x, per_y, lane, last);
@@ -521,8 +569,8 @@ int launch_noise_fbm(float       *out,
   dim3 blk(64, 4);
   dim3 grd((nx + 63) / 64, (ny + 3) / 4);
 #define HMG_FBM(T) k_noise_fbm<T><<<grd, blk, 0, st>>>(out, ctrl, noise_x, noise_y, a)
-  // without warping planes, every type but SIMPLEX2 shares its lattice
-  // values across each wavefront
+  // without warping planes, every type shares its lattice values across
+  // each wavefront
   const bool wave = !noise_x && !noise_y;
 #define HMG_FBM_WAVE(T)                                                                \
   if (wave)                                                                            \
@@ -534,7 +582,7 @@ int launch_noise_fbm(float       *out,
   case 1: HMG_FBM_WAVE(1); break;
   case 2: HMG_FBM_WAVE(2); break;
   case 3: HMG_FBM_WAVE(3); break;
-  case 4: HMG_FBM(4); break;
+  case 4: HMG_FBM_WAVE(4); break;
   case 6: HMG_FBM_WAVE(6); break;
   case 7: HMG_FBM_WAVE(7); break;
   case 9: HMG_FBM_WAVE(9); break;

@@ -13,6 +13,7 @@
 
 #include <math.h>
 #include <float.h>
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -78,7 +79,10 @@ static float orc_reduce(float x, int *quadrant)
   float r = fmaf(-k, ORC_PIO2_HI, x);
   r = fmaf(-k, ORC_PIO2_MID, r);
   r = fmaf(-k, ORC_PIO2_LO, r);
-  *quadrant = ((int)k) & 3;
+  /* k mod 4 for every k: floats of magnitude 2^25 and above are multiples
+   * of 4, so clamping there keeps the quadrant and keeps the conversion in
+   * range (a NaN k gives 0). */
+  *quadrant = ((int)fminf(fmaxf(k, -0x1p25f), 0x1p25f)) & 3;
   return r;
 }
 
@@ -186,10 +190,16 @@ static float orc_wrap(float c, int period)
   return c;
 }
 
-/* scale_period (noise.cl:21-26). */
+/* scale_period (noise.cl:21-26), the float-to-int conversion saturating
+ * (NaN to 0, beyond the int range to INT_MIN / INT_MAX) so that periods
+ * past 2^31 after many octaves are defined. */
 static int orc_scale_period(int period, float lacunarity)
 {
-  if (period > 0) period = (int)((float)period * lacunarity + 0.5f);
+  if (period > 0)
+  {
+    const float p = (float)period * lacunarity + 0.5f;
+    period = p != p ? 0 : p >= 0x1p31f ? INT_MAX : p < -0x1p31f ? INT_MIN : (int)p;
+  }
   return period;
 }
 
@@ -805,6 +815,15 @@ float orc_powf(float x, float y)
 void orc_powf_many(const float *x, const float *y, float *out, long n)
 {
   for (long i = 0; i < n; i++) out[i] = orc_powf(x[i], y[i]);
+}
+
+void orc_sincos_many(const float *x, float *s, float *c, long n)
+{
+  for (long i = 0; i < n; i++)
+  {
+    s[i] = orc_sinf(x[i]);
+    c[i] = orc_cosf(x[i]);
+  }
 }
 
 static float orc_powf_disch(float x, float e)

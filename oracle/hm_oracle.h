@@ -228,6 +228,8 @@ float orc_expf(float x);
 /* the contract's pow (FP32 only, shared with the kernels), see hm_oracle.c */
 float orc_powf(float x, float y);
 void  orc_powf_many(const float *x, const float *y, float *out, long n);
+/* orc_sinf and orc_cosf elementwise */
+void  orc_sincos_many(const float *x, float *s, float *c, long n);
 
 /* hmap::gpu::snow_simulation: the final snow depth into s (nx*ny). */
 void orc_snow_simulation_f32(const float *z, int nx, int ny, float snow_depth,

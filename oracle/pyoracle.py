@@ -152,6 +152,7 @@ def lib():
     L.orc_powf.argtypes = [fl, fl]
     L.orc_powf.restype = fl
     L.orc_powf_many.argtypes = [F32P, F32P, F32P, C.c_long]
+    L.orc_sincos_many.argtypes = [F32P, F32P, F32P, C.c_long]
     snow_tail = [C.c_int] + [fl] * 7 + [C.c_int, fl]
     L.orc_snow_simulation_f32.argtypes = ([F32P, C.c_int, C.c_int, fl, F32P, F32P, F32P]
                                           + snow_tail + [F32P])
@@ -402,6 +403,15 @@ def expf(x):
 def powf(x, y):
     """The FP32 contract's pow (shared with the kernels)."""
     return float(lib().orc_powf(float(x), float(y)))
+
+
+def sincos_many(x):
+    """(orc_sinf, orc_cosf) elementwise over a float32 array."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().orc_sincos_many(x, s, c, x.size)
+    return s, c
 
 
 def powf_many(x, y):

@@ -3,8 +3,8 @@ float pattern they can receive, on the GPU (tests/cpp/arith_check.cpp):
 the square root without small-input scaling (bit-equal to sqrtf from 2^-96
 up, below 2^-47 under it), its wave-checked form (bit-equal everywhere), and
 the float form of the fixed-point volume term (equal to the integer form on
-[0, 1)); and the kernels' pow against the oracle's copy, bit for bit
-(tests/cpp/pow_check.cpp)."""
+[0, 1)); and the kernels' pow, sin and cos against the oracle's copies,
+bit for bit (tests/cpp/pow_check.cpp)."""
 import os
 import subprocess
 import tempfile
@@ -64,3 +64,39 @@ def test_pow_device_equals_oracle(orc):
     bad = np.flatnonzero(~same)
     assert bad.size == 0, "%d of %d differ, first x=%r y=%r device %r oracle %r" % (
         bad.size, x.size, x[bad[0]], y[bad[0]], got[bad[0]], want[bad[0]])
+
+
+def _sincos_inputs():
+    """Every 1031st float pattern from +0 to +inf and their negatives (the
+    reduction's quadrant beyond 2^31 pi/2 included), every 13th pattern of
+    [3e9, 1e10] where a quadrant taken by an out-of-range conversion would
+    show, a dense sweep of [-8 pi, 8 pi], and the special values."""
+    pat = np.arange(0, 0x7f800001, 1031, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    lo, hi = np.array([3e9, 1e10], dtype=np.float32).view(np.uint32)
+    band = np.arange(lo, hi, 13, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    small = np.linspace(-8 * np.pi, 8 * np.pi, 200001, dtype=np.float32)
+    specials = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 2.0 ** 25, -2.0 ** 25, 2.0 ** 31,
+                         3.4e38], dtype=np.float32)
+    return np.concatenate([pat, -pat, band, -band, small, specials])
+
+
+def test_sincos_device_equals_oracle(orc):
+    exe = os.path.join(ROOT, "tests", "cpp", "pow_check")
+    assert os.path.exists(exe), "tests/cpp/pow_check not built (run the build first)"
+    x = _sincos_inputs()
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([x.size], dtype=np.int32).tofile(f)
+            x.tofile(f)
+            x.tofile(f)
+        r = subprocess.run([exe, fin, fout, "sincos"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        got = np.fromfile(fout, dtype=np.float32).reshape(3, x.size)
+    ws, wc = orc.sincos_many(x)
+    for name, g, w in (("hm_sin", got[0], ws), ("hm_sincos sin", got[1], ws),
+                       ("hm_sincos cos", got[2], wc)):
+        same = (g.view(np.uint32) == w.view(np.uint32)) | (np.isnan(g) & np.isnan(w))
+        bad = np.flatnonzero(~same)
+        assert bad.size == 0, "%s: %d of %d differ, first x=%r device %r oracle %r" % (
+            name, bad.size, x.size, x[bad[0]], g[bad[0]], w[bad[0]])

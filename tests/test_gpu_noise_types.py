@@ -52,15 +52,17 @@ def test_noise_fbm_large_simplex_and_worley(hm, orc):
         assert np.array_equal(g, o), t
 
 
-# Without warping planes every type but SIMPLEX2 shares each wavefront's
-# lattice values (64 cells of one row; Perlin and VALUE / VALUE_LINEAR at
-# most 31 lattice columns, VALUE_CUBIC 13, WORLEY 19; |ix| < 2^23); other
-# waves compute per cell.  Both must give the oracle's bits: partial
+# Without warping planes every type shares each wavefront's lattice values
+# (64 cells of one row; Perlin and VALUE / VALUE_LINEAR at most 31 lattice
+# columns, VALUE_CUBIC 13, WORLEY 19, SIMPLEX2 15 columns and 3 rows of its
+# skewed lattice; |index| < 2^23); other waves compute per cell.  Both must give the oracle's bits: partial
 # wavefronts at the right edge, periods, a reversed bounding box and a
 # negative wavenumber (lattice index decreasing along the row), frequencies
 # where only some octaves fit the shared form, spans on either side of the
-# VALUE_CUBIC and WORLEY limits, huge coordinates, and a negative lacunarity.
-@pytest.mark.parametrize("noise_type", [1, 2, 3, 6, 7, 9, 10])
+# VALUE_CUBIC and WORLEY limits, huge coordinates (hash arguments past
+# 2^31 pi/2, where sin's quadrant must still be k mod 4), periods that
+# saturate past 2^31 after an octave, and a negative lacunarity.
+@pytest.mark.parametrize("noise_type", [1, 2, 3, 4, 6, 7, 9, 10])
 @pytest.mark.parametrize("case", [
     dict(shape=(130, 77), kw=(4.0, 6.0), octaves=5, bbox=(-0.5, 1.5, 0.25, 2.0), period=(4, 6)),
     dict(shape=(193, 5), kw=(5.0, 3.0), octaves=8, bbox=(1.0, 0.0, 1.0, 0.0), period=(0, 0)),
@@ -76,6 +78,8 @@ def test_noise_fbm_large_simplex_and_worley(hm, orc):
     dict(shape=(128, 6), kw=(19.0, 3.0), octaves=2, bbox=(0.25, 1.25, 0.0, 1.0), period=(5, 3)),
     dict(shape=(64, 4), kw=(19.0, 2.0), octaves=1, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
     dict(shape=(64, 4), kw=(19.5, 2.0), octaves=1, bbox=(0.0, 1.0, 0.0, 1.0), period=(0, 0)),
+    dict(shape=(130, 7), kw=(4.0, 4.0), octaves=3, bbox=(-2.0, 1.0, -1.0, 1.0),
+         period=(2 ** 30, 2 ** 29 + 1)),
 ])
 def test_noise_fbm_shared_lattice(hm, orc, noise_type, case):
     nx, ny = case["shape"]

@@ -64,6 +64,25 @@ def test_project_sin_odd_cos_even_exactly():
         assert L.orc_cosf(float(-x)) == L.orc_cosf(float(x))
 
 
+def test_project_sin_cos_symmetric_at_every_magnitude():
+    """Beyond 2^22 the reduction is no longer accurate (the hashes only need
+    it deterministic), but its quadrant is k mod 4 for every k, so sin stays
+    odd and cos even bit for bit over the whole float range: a quadrant
+    taken by a saturating or wrapping float-to-int conversion (3 for k at
+    or past 2^31, 0 for -k) breaks the symmetry from 2^31 pi/2 up.
+    Every 997th float pattern from the smallest subnormal (zero aside: the
+    reduction's fma turns -0 into +0), and a dense band over [3e9, 1e10]."""
+    pat = np.arange(1, 0x7f800000, 997, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    lo, hi = np.array([3e9, 1e10], dtype=np.float32).view(np.uint32)
+    band = np.arange(lo, hi, 7, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    x = np.concatenate([pat, band])
+    s, c = O.sincos_many(x)
+    sn, cn = O.sincos_many(-x)
+    ok = lambda a, b: (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))
+    bad = np.flatnonzero(~(ok(sn, -s) & ok(cn, c)))
+    assert bad.size == 0, (x[bad[:5]], s[bad[:5]], sn[bad[:5]])
+
+
 def test_fract_matches_opencl_definition():
     L = O.lib()
     assert L.orc_fractf(-0.25) == 0.75
