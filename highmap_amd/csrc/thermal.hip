@@ -20,8 +20,11 @@
 //  k_thermal_stream2 two Jacobi steps per launch with a uniform talus, the
 //                    second from the first's rows kept in registers (calls of
 //                    two or more steps from 2^18 cells; 4 B per cell-step).
+//                    With PLANE, each cell forms its eight exchanges with its
+//                    own talus: a talus plane (12 B/cell) or a negative
+//                    scalar.
 //  k_thermal_step    one Jacobi step, one cell per lane, 9-point stencil, for
-//                    a talus plane (12 B/cell) or thermal_with_bedrock.
+//                    thermal_with_bedrock and grids below 4 x 4.
 //  k_thermal_blocked T Jacobi steps per launch on an LDS tile with a T-cell
 //                    halo (temporal blocking), for grids below 2^18 cells
 //                    where launches dominate.
@@ -456,12 +459,73 @@ __device__ __forceinline__ float th_border(
   return out;
 }
 
-template <int P, bool EXTRAP>
+// One Jacobi step of a row with each cell's own talus t (a talus plane, or
+// a scalar of any sign): all eight exchanges of the cell, each
+// copysign(max(0.2 (|d| - c t), 0), other - self) (/ 1.414 on diagonals),
+// which equals the reference's exchange for every t (a negative one
+// included: every pair is then over the threshold, and other == self gives
+// +m as the reference's else branch does) and for NaNs (0); the terms in the
+// reference's order, the sum started as 0 + term (never -0).  zm, z0, z1 are
+// rows r-1, r, r+1; the lane exchanges fold into the arithmetic.
+__device__ __forceinline__ float th_cell_step(float zm, float z0, float z1, float t)
+{
+  constexpr float B = 1.414f;
+  constexpr float R = 0x1.6a17e6p-1f; // RN(1 / 1.414f)
+  const float     md = t, mdd = B * t;
+  const float     n[8] = {th_left(z0), z1,          zm,           th_right(z0),
+                          th_left(zm), th_left(z1), th_right(zm), th_right(z1)};
+  float           e[8];
+#pragma unroll
+  for (int k = 0; k < 4; k++) e[k] = th_pair_axial(z0, n[k], md);
+  // diagonals: m / 1.414 by div_1414's product and fma correction when
+  // every lane's m is 0 or in [2^-100, inf) (m >= 0: as integers, the
+  // minimum of bits - 1 and the maximum of bits bound all four), else by
+  // the IEEE division for the whole wave
+  float    nd[4], m[4];
+  uint32_t lo = 0xffffffffu, hi = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+  {
+    nd[k] = n[4 + k] - z0;
+    m[k] = fmaxf(0.2f * (fabsf(nd[k]) - mdd), 0.f);
+    const uint32_t b = __float_as_uint(m[k]);
+    lo = min(lo, b - 1u);
+    hi = max(hi, b);
+  }
+  const bool odd = lo < 0x0d7fffffu || hi >= 0x7f800000u;
+  if (__builtin_expect(__any(odd), 0))
+  {
+#pragma unroll
+    for (int k = 0; k < 4; k++) e[4 + k] = copysignf(m[k] / B, nd[k]);
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+    {
+      const float q0 = m[k] * R;
+      e[4 + k] = copysignf(fmaf(fmaf(-q0, B, m[k]), R, q0), nd[k]);
+    }
+  }
+  float amount = 0.f + e[0];
+  asm volatile("" : "+v"(amount));
+#pragma unroll
+  for (int k = 1; k < 8; k++) amount = amount + e[k];
+  float step = z0 + amount;
+  asm volatile("" : "+v"(step));
+  return step;
+}
+
+// PLANE: the exchanges per cell with its own talus (th_cell_step): a talus
+// plane (talus, 4 more bytes per cell, its rows loaded with the heights) or,
+// with talus null, a negative scalar ts.
+template <int P, bool EXTRAP, bool PLANE>
 __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict__ z,
                                                         float *__restrict__ zn,
                                                         int   nx,
                                                         int   ny,
                                                         float ts,
+                                                        const float *__restrict__ talus,
                                                         int   strips,
                                                         int   chunk_rows,
                                                         int   n_items)
@@ -493,14 +557,26 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
     const float *row = z + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
     return at_off(row, off);
   };
+  auto load_t = [&](int r) {
+    if (!talus) return ts;
+    const float *row = talus + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
+    return at_off(row, off);
+  };
 
   constexpr int U = P + 3; // rows r-1 .. r+P+1
   float         R[U];
+  float         TR[U]; // PLANE: the talus of the rows in R
   // pre-roll: rows j0-1 .. j0+P, and the exchanges of row j0-1 with row j0
 #pragma unroll
   for (int q = 0; q <= P + 1; q++) R[q] = load(j0 - 1 + q);
-  float pU, pUR, pUL; // exchanges of the previous row with the row above it
-  th_row_pairs(R[0], R[1], md, mdd, pU, pUR, pUL);
+  if (PLANE)
+  {
+    TR[0] = ts; // row j0 - 1: no step of it here
+#pragma unroll
+    for (int q = 1; q <= P + 1; q++) TR[q] = load_t(j0 - 1 + q);
+  }
+  float pU = 0.f, pUR = 0.f, pUL = 0.f; // exchanges of the previous row with the row above it
+  if (!PLANE) th_row_pairs(R[0], R[1], md, mdd, pU, pUR, pUL);
   float f1 = 0.f, f2 = 0.f; // EXTRAP: final values of rows r-1 and r-2
 
   for (int r0 = j0; r0 < j1; r0 += U)
@@ -512,9 +588,14 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
       if (r >= j1) break;
       const float zm = R[q % U], z0 = R[(q + 1) % U], z1 = R[(q + 2) % U];
       // the row P + 1 below the next one, into the slot of row r - 1
-      if (r + P + 1 <= j1) R[(q + P + 2) % U] = load(r + P + 1);
+      if (r + P + 1 <= j1)
+      {
+        R[(q + P + 2) % U] = load(r + P + 1);
+        if (PLANE) TR[(q + P + 2) % U] = load_t(r + P + 1);
+      }
 
-      const float step = th_row_step(z0, z1, md, mdd, pU, pUR, pUL);
+      const float step = PLANE ? th_cell_step(zm, z0, z1, TR[(q + 1) % U])
+                               : th_row_step(z0, z1, md, mdd, pU, pUR, pUL);
 
       float *row = zn + (size_t)r * (size_t)nx;
       if (!EXTRAP)
@@ -661,13 +742,20 @@ static int launch_thermal_stream2(const float *a, float *b, int nx, int ny, floa
   return check_launch("thermal_stream2");
 }
 
-template <int P, bool EXTRAP>
-static int launch_thermal_stream(const float *a, float *b, int nx, int ny, float ts, int num_cus,
-                                 hipStream_t st)
+template <int P, bool EXTRAP, bool PLANE = false>
+static int launch_thermal_stream(const float *a,
+                                 float       *b,
+                                 int          nx,
+                                 int          ny,
+                                 float        ts,
+                                 int          num_cus,
+                                 hipStream_t  st,
+                                 const float *talus = nullptr)
 {
   static const int blocks_per_cu = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream<P, EXTRAP>, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream<P, EXTRAP, PLANE>, 256,
+                                                     0) !=
             hipSuccess ||
         n < 1)
     {
@@ -685,8 +773,8 @@ static int launch_thermal_stream(const float *a, float *b, int nx, int ny, float
   const long chunks = (ny + ch - 1) / ch;
   const long items = strips * chunks;
   const int  G = (int)((items + 3) / 4);
-  k_thermal_stream<P, EXTRAP><<<G, 256, 0, st>>>(a, b, nx, ny, ts, (int)strips, (int)ch,
-                                                 (int)items);
+  k_thermal_stream<P, EXTRAP, PLANE>
+      <<<G, 256, 0, st>>>(a, b, nx, ny, ts, talus, (int)strips, (int)ch, (int)items);
   return check_launch("thermal_stream");
 }
 
@@ -739,6 +827,9 @@ int launch_thermal(hmg_ctx     *ctx,
     }
   }
   const bool stream = uniform;
+  // a talus plane or a negative scalar, without bedrock: the streaming
+  // kernel with each cell's own talus
+  const bool plane_stream = !uniform && !bedrock && nx >= 4 && ny >= 4;
   dim3       blk(64, 4);
   dim3       grd((nx + 63) / 64, (ny + 3) / 4);
   // two steps per launch while two or more are left (an odd last step, or
@@ -766,6 +857,13 @@ int launch_thermal(hmg_ctx     *ctx,
     }
     else if (stream)
       HMG_TRY((launch_thermal_stream<2, false>(a, b, nx, ny, talus_scalar, cus, st)));
+    else if (plane_stream && last && extrapolate)
+    {
+      HMG_TRY((launch_thermal_stream<2, true, true>(a, b, nx, ny, talus_scalar, cus, st, talus)));
+      borders_done = true;
+    }
+    else if (plane_stream)
+      HMG_TRY((launch_thermal_stream<2, false, true>(a, b, nx, ny, talus_scalar, cus, st, talus)));
     else
     {
       k_thermal_step<<<grd, blk, 0, st>>>(a, b, nx, ny, talus, talus_scalar, bedrock);

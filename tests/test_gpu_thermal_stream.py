@@ -7,7 +7,10 @@ kernels;
 the shapes cover one strip (nx <= 62), the strip edge (62, 63, 64, 124,
 125, 126), the moved last strip, grids of 3 to 6 rows, the last chunk
 shorter than 3 rows, zero talus, a talus no slope exceeds and signed
-zeros."""
+zeros.  Talus planes and negative scalars take the streaming kernel with
+each cell's own talus (all eight exchanges per cell) at every size from
+4 x 4: the same shapes, planes with zeros, tiny, negative and huge values,
+NaNs and infinities in the heights, and signed zeros."""
 import numpy as np
 import pytest
 
@@ -104,3 +107,55 @@ def test_default_choice_above_2_18_cells(hm, orc, iters):
     z = z0.copy()
     hm.thermal(z, 2.0 / 1500, iters)
     assert _bits_equal(z, orc.thermal(z0, 2.0 / 1500, iters))
+
+
+def _plane(nx, ny, seed, scale):
+    r = np.random.default_rng(seed)
+    t = (r.random((ny, nx), dtype=np.float32) * np.float32(scale)).astype(np.float32)
+    t[r.random((ny, nx)) < 0.1] = 0.0
+    t[r.random((ny, nx)) < 0.05] = -np.float32(scale)
+    t[r.random((ny, nx)) < 0.02] = np.float32(1e-35)
+    return t
+
+
+@pytest.mark.parametrize("nx,ny", SHAPES)
+@pytest.mark.parametrize("iters", [1, 2, 5])
+def test_plane_stream_steps_match_oracle(hm, orc, nx, ny, iters):
+    z0 = _input(orc, nx, ny, nx * 1000 + ny + 7)
+    t = _plane(nx, ny, nx + ny, 3.0 / nx)
+    z = z0.copy()
+    hm.thermal(z, t, iters)
+    assert _bits_equal(z, orc.thermal(z0, t, iters))
+    for neg in (-1e-3, -2.0 / nx):
+        z = z0.copy()
+        hm.thermal(z, neg, iters)
+        assert _bits_equal(z, orc.thermal(z0, neg, iters)), neg
+
+
+def test_plane_stream_special_values(hm, orc):
+    """Heights with NaN, +-inf, huge opposite values (differences that
+    overflow), tiny differences around the talus (quotients below 2^-100)
+    and -0.0; a plane holding NaN and +-inf talus.  The call delivers the
+    result and reports it as non-finite (-4, HMG_ERR_NONFINITE)."""
+    nx, ny = 131, 45
+    z0 = orc.fbm_terrain(nx, ny, 11)
+    z0[10, 20] = np.nan
+    z0[30, 70] = np.inf
+    z0[31, 71] = -np.inf
+    z0[5, 100] = 3e38
+    z0[5, 101] = -3e38
+    z0[40, 5:60] = -0.0
+    z0[20, 40:44] = np.float32(1e-31) * np.arange(4, dtype=np.float32)
+    t = _plane(nx, ny, 3, 2.0 / nx)
+    t[20, 40:44] = np.float32(1e-31)
+    t[12, 12] = np.nan
+    t[13, 13] = np.inf
+    t[14, 14] = -np.inf
+    for it in (1, 3):
+        z = z0.copy()
+        with pytest.raises(hm.HmgError) as e:
+            hm.thermal(z, t, it)
+        assert e.value.code == -4
+        want = orc.thermal(z0, t, it)
+        same = (z.view(np.uint32) == want.view(np.uint32)) | (np.isnan(z) & np.isnan(want))
+        assert same.all(), (it, np.argwhere(~same)[:5])
