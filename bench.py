@@ -661,6 +661,44 @@ def thermal256_checks(z0, z):
             "interior_sum_drift_fp64": (interior_sum(o64) - s0) / s0}
 
 
+def run_thermal(args):
+    """hmap::gpu::thermal at n x n, device-resident (fBm seed 1 generated on
+    the device): a uniform talus 1/n and a talus plane (uniform random in
+    [0, 2/n), torch's generator, seed 5), 1 and 50 steps per call; HIP
+    events around the call on the library's stream, best of 3.  GB/s counts
+    the algorithmic bytes per cell-step: 8 (z read and written) with a
+    scalar, 12 (and the talus read) with a plane."""
+    torch, hm, ctx = _gpu()
+    n = args.n
+    z0 = device_terrain(torch, hm, ctx, n, n, 1)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    plane = torch.rand((n, n), device="cuda", generator=g) * (2.0 / n)
+    z = torch.empty_like(z0)
+    st = torch.cuda.ExternalStream(ctx.stream)
+    out = {"metric": "Gcell-iter/s thermal %dx%d (device-resident)" % (n, n), "unit": "Gcell-iter/s",
+           "n_gpus": 1}
+    for name, tp, ts, bpc in (("uniform", None, 1.0 / n, 8), ("plane", plane.data_ptr(), 0.0, 12)):
+        for steps in (1, 50):
+            best = None
+            for _ in range(3):
+                z.copy_(z0)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                hm._check(hm.lib().hmg_thermal_dev(ctx.handle, z.data_ptr(), n, n, tp, ts, steps,
+                                                   None))
+                e1.record(st)
+                e1.synchronize()
+                ms = e0.elapsed_time(e1)
+                best = ms if best is None else min(best, ms)
+            rate = n * n * steps / (best / 1e3) / 1e9
+            out["%s_x%d" % (name, steps)] = {"us_per_step": round(best * 1e3 / steps, 2),
+                                             "gcell_iter_s": round(rate, 2),
+                                             "gbs": round(rate * bpc, 1)}
+    out["value"] = out["uniform_x50"]["gcell_iter_s"]
+    return out
+
+
 def run_thermal256(args, cpu=True, big_grid=True, check=True):
     """hmap::gpu::thermal drop-in on 256^2 fBm (seed 1), talus 2/256, 50
     iterations: host-pointer call (transfers included) and device-resident,
@@ -1120,8 +1158,8 @@ def main():
                     help="serve: cells of the jobs that run at once (0: no cap)")
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "batch", "batchforms", "serve", "precision",
-                             "thermal256", "vpipes1024", "flow", "snow", "tiles", "copy",
-                             "ceiling", "noise"])
+                             "thermal", "thermal256", "vpipes1024", "flow", "snow", "tiles",
+                             "copy", "ceiling", "noise"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline: skip the BF16/FP16 kernel rates, the thermal 256^2 and "
@@ -1138,7 +1176,8 @@ def main():
         sys.exit(2)
     fn = {"headline": run_headline, "batch": run_batch, "batchforms": run_batch_forms,
           "precision": run_precision,
-          "thermal256": run_thermal256, "vpipes1024": run_vpipes1024, "flow": run_flow,
+          "thermal": run_thermal, "thermal256": run_thermal256, "vpipes1024": run_vpipes1024,
+          "flow": run_flow,
           "snow": run_snow, "tiles": run_tiles, "copy": run_copy, "ceiling": run_ceiling,
           "noise": run_noise}
     if args.workload == "serve":
