@@ -644,12 +644,16 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
 
 constexpr int TH2_OUT = 60; // columns written per wavefront
 
-template <int P>
+// PLANE: each cell's own talus (th_cell_step), from a talus plane (rows
+// loaded with the heights, a ring beside theirs) or, with talus null, a
+// negative scalar ts.
+template <int P, bool PLANE>
 __global__ void __launch_bounds__(256) k_thermal_stream2(const float *__restrict__ z,
                                                          float *__restrict__ zn,
                                                          int   nx,
                                                          int   ny,
                                                          float ts,
+                                                         const float *__restrict__ talus,
                                                          int   strips,
                                                          int   chunk_rows,
                                                          int   n_items)
@@ -672,25 +676,43 @@ __global__ void __launch_bounds__(256) k_thermal_stream2(const float *__restrict
     const float *row = z + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
     return at_off(row, off);
   };
+  auto load_t = [&](int r) {
+    if (!talus) return ts;
+    const float *row = talus + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
+    return at_off(row, off);
+  };
 
   // rings of U slots indexed by compile-time constants (rows rotate by
-  // renaming): with step-1 row r, Z holds input rows r-1 .. r+P+1 and S1
-  // step-1 rows r-2 .. r
+  // renaming): with step-1 row r, Z holds input rows r-1 .. r+P+1 (TR
+  // their talus, PLANE) and S1 step-1 rows r-2 .. r
   constexpr int U = P + 3;
-  float         Z[U], S1[U];
-  float         pU, pUR, pUL; // step 1: exchanges of input row r-1 with row r
-  float         qU, qUR, qUL; // step 2: exchanges of step-1 row r-2 with row r-1
+  float         Z[U], S1[U], TR[U];
+  float         pU = 0.f, pUR = 0.f, pUL = 0.f; // step 1: exchanges of input row r-1 with row r
+  float         qU = 0.f, qUR = 0.f, qUL = 0.f; // step 2: exchanges of step-1 row r-2 with row r-1
   {
     // pre-roll: step 1 of rows j0-1 and j0, step 2's exchanges between them
     const float za = load(j0 - 2), zb = load(j0 - 1);
 #pragma unroll
     for (int q = 0; q <= P + 1; q++) Z[q] = load(j0 + q);
-    th_row_pairs(za, zb, md, mdd, pU, pUR, pUL);
-    float s = th_row_step(zb, Z[0], md, mdd, pU, pUR, pUL);
-    S1[0] = th_border(s, za, zb, Z[0], j0 - 1, ny, edge_strip, c0, cn);
-    s = th_row_step(Z[0], Z[1], md, mdd, pU, pUR, pUL);
-    S1[1] = th_border(s, zb, Z[0], Z[1], j0, ny, edge_strip, c0, cn);
-    th_row_pairs(S1[0], S1[1], md, mdd, qU, qUR, qUL);
+    if (PLANE)
+    {
+#pragma unroll
+      for (int q = 0; q <= P + 1; q++) TR[q] = load_t(j0 + q);
+      TR[U - 1] = load_t(j0 - 1); // row j0 - 1, its slot in the ring
+      float s = th_cell_step(za, zb, Z[0], TR[U - 1]);
+      S1[0] = th_border(s, za, zb, Z[0], j0 - 1, ny, edge_strip, c0, cn);
+      s = th_cell_step(zb, Z[0], Z[1], TR[0]);
+      S1[1] = th_border(s, zb, Z[0], Z[1], j0, ny, edge_strip, c0, cn);
+    }
+    else
+    {
+      th_row_pairs(za, zb, md, mdd, pU, pUR, pUL);
+      float s = th_row_step(zb, Z[0], md, mdd, pU, pUR, pUL);
+      S1[0] = th_border(s, za, zb, Z[0], j0 - 1, ny, edge_strip, c0, cn);
+      s = th_row_step(Z[0], Z[1], md, mdd, pU, pUR, pUL);
+      S1[1] = th_border(s, zb, Z[0], Z[1], j0, ny, edge_strip, c0, cn);
+      th_row_pairs(S1[0], S1[1], md, mdd, qU, qUR, qUL);
+    }
   }
 
   for (int r0 = j0 + 1; r0 <= j1; r0 += U)
@@ -701,28 +723,41 @@ __global__ void __launch_bounds__(256) k_thermal_stream2(const float *__restrict
       const int r = r0 + q;
       if (r > j1) break;
       const float zm = Z[q % U], z0 = Z[(q + 1) % U], z1 = Z[(q + 2) % U];
+      // step 1 of row r and step 2 of row r-1 take the talus of rows r and
+      // r-1 (PLANE), in slots the load below does not replace
+      const float tr = TR[(q + 1) % U], trm = TR[q % U];
       // input row r+P+1 into the slot of row r-2 (the last needed is j1+1)
-      if (r + P + 1 <= j1 + 1) Z[(q + P + 2) % U] = load(r + P + 1);
+      if (r + P + 1 <= j1 + 1)
+      {
+        Z[(q + P + 2) % U] = load(r + P + 1);
+        if (PLANE) TR[(q + P + 2) % U] = load_t(r + P + 1);
+      }
       // step 1 of row r
-      float s = th_row_step(z0, z1, md, mdd, pU, pUR, pUL);
+      float s = PLANE ? th_cell_step(zm, z0, z1, tr) : th_row_step(z0, z1, md, mdd, pU, pUR, pUL);
       s = th_border(s, zm, z0, z1, r, ny, edge_strip, c0, cn);
       S1[(q + 2) % U] = s;
       // step 2 of row r-1
       const float a = S1[q % U], b = S1[(q + 1) % U];
-      float       o = th_row_step(b, s, md, mdd, qU, qUR, qUL);
+      float       o = PLANE ? th_cell_step(a, b, s, trm) : th_row_step(b, s, md, mdd, qU, qUR, qUL);
       o = th_border(o, a, b, s, r - 1, ny, edge_strip, c0, cn);
       if (writer) at_off(zn + (size_t)(r - 1) * (size_t)nx, soff) = o;
     }
   }
 }
 
-template <int P>
-static int launch_thermal_stream2(const float *a, float *b, int nx, int ny, float ts, int num_cus,
-                                  hipStream_t st)
+template <int P, bool PLANE = false>
+static int launch_thermal_stream2(const float *a,
+                                  float       *b,
+                                  int          nx,
+                                  int          ny,
+                                  float        ts,
+                                  int          num_cus,
+                                  hipStream_t  st,
+                                  const float *talus = nullptr)
 {
   static const int blocks_per_cu = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream2<P>, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream2<P, PLANE>, 256, 0) !=
             hipSuccess ||
         n < 1)
     {
@@ -738,7 +773,8 @@ static int launch_thermal_stream2(const float *a, float *b, int nx, int ny, floa
   const long chunks = (ny + ch - 1) / ch;
   const long items = strips * chunks;
   const int  G = (int)((items + 3) / 4);
-  k_thermal_stream2<P><<<G, 256, 0, st>>>(a, b, nx, ny, ts, (int)strips, (int)ch, (int)items);
+  k_thermal_stream2<P, PLANE>
+      <<<G, 256, 0, st>>>(a, b, nx, ny, ts, talus, (int)strips, (int)ch, (int)items);
   return check_launch("thermal_stream2");
 }
 
@@ -835,10 +871,13 @@ int launch_thermal(hmg_ctx     *ctx,
   // two steps per launch while two or more are left (an odd last step, or
   // a lone one, runs below; after an even count the borders are
   // extrapolated by their own launches)
-  const bool pairs = stream && T != 0 && nx >= 4 && ny >= 4;
+  const bool pairs = (stream || plane_stream) && T != 0 && nx >= 4 && ny >= 4;
   while (pairs && iterations - done >= 2)
   {
-    HMG_TRY((launch_thermal_stream2<2>(a, b, nx, ny, talus_scalar, cus, st)));
+    if (plane_stream)
+      HMG_TRY((launch_thermal_stream2<2, true>(a, b, nx, ny, talus_scalar, cus, st, talus)));
+    else
+      HMG_TRY((launch_thermal_stream2<2>(a, b, nx, ny, talus_scalar, cus, st)));
     float *t = a;
     a = b;
     b = t;

@@ -7,9 +7,9 @@ kernels;
 the shapes cover one strip (nx <= 62), the strip edge (62, 63, 64, 124,
 125, 126), the moved last strip, grids of 3 to 6 rows, the last chunk
 shorter than 3 rows, zero talus, a talus no slope exceeds and signed
-zeros.  Talus planes and negative scalars take the streaming kernel with
-each cell's own talus (all eight exchanges per cell) at every size from
-4 x 4: the same shapes, planes with zeros, tiny, negative and huge values,
+zeros.  Talus planes and negative scalars take the streaming kernels with
+each cell's own talus (all eight exchanges per cell; one or two steps per
+launch as above) at every size from 4 x 4: the same shapes, planes with zeros, tiny, negative and huge values,
 NaNs and infinities in the heights, and signed zeros."""
 import numpy as np
 import pytest
@@ -120,7 +120,7 @@ def _plane(nx, ny, seed, scale):
 
 @pytest.mark.parametrize("nx,ny", SHAPES)
 @pytest.mark.parametrize("iters", [1, 2, 5])
-def test_plane_stream_steps_match_oracle(hm, orc, nx, ny, iters):
+def test_plane_stream_steps_match_oracle(hm, orc, streaming, nx, ny, iters):
     z0 = _input(orc, nx, ny, nx * 1000 + ny + 7)
     t = _plane(nx, ny, nx + ny, 3.0 / nx)
     z = z0.copy()
@@ -132,7 +132,7 @@ def test_plane_stream_steps_match_oracle(hm, orc, nx, ny, iters):
         assert _bits_equal(z, orc.thermal(z0, neg, iters)), neg
 
 
-def test_plane_stream_special_values(hm, orc):
+def test_plane_stream_special_values(hm, orc, streaming):
     """Heights with NaN, +-inf, huge opposite values (differences that
     overflow), tiny differences around the talus (quotients below 2^-100)
     and -0.0; a plane holding NaN and +-inf talus.  The call delivers the
