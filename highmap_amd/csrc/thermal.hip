@@ -17,9 +17,11 @@
 //                    once for the pair of cells it joins; the last step of a
 //                    thermal() call also applies extrapolate_borders.
 //                    8 B/cell (z read + write).
-//  k_thermal_stream2 two Jacobi steps per launch with a uniform talus, the
-//                    second from the first's rows kept in registers (calls of
-//                    two or more steps from 2^18 cells; 4 B per cell-step).
+//  k_thermal_stream2 two Jacobi steps per launch, the second from the first's
+//                    rows kept in registers: a uniform talus (calls of two or
+//                    more steps from 2^18 cells; 4 B per cell-step) or, with
+//                    PLANE, each cell's own talus (from 4 x 4; 6 B per
+//                    cell-step with a plane).
 //                    With PLANE, each cell forms its eight exchanges with its
 //                    own talus: a talus plane (12 B/cell) or a negative
 //                    scalar.
