@@ -32,7 +32,10 @@
 //                    where launches dominate.
 //  k_extrap_cols / k_extrap_rows  extrapolate_borders, columns then rows,
 //                    where it is not folded into the last step.
-//  k_thermal_auto_step  one thermal_auto_bedrock Jacobi step.
+//  k_thermal_auto_stream  one thermal_auto_bedrock Jacobi step, streamed
+//                    (each cell's own talus; the bedrock plane updated in
+//                    place by the owner of each column), from 4 x 4.
+//  k_thermal_auto_step  the same step one cell per lane, below 4 x 4.
 //  k_thermal_post    the deposition map and the mask blend after a call.
 //  k_sd_pre / k_sd_post / k_sd_finish  sediment_deposition's element-wise
 //                    rounds around its thermal sub-iterations.
@@ -816,6 +819,124 @@ static int launch_thermal_stream(const float *a,
   return check_launch("thermal_stream");
 }
 
+// thermal_auto_bedrock (thermal.cl:102-145), one Jacobi step, streamed as
+// k_thermal_stream<P, false, true>: each cell's eight exchanges with its own
+// talus (th_cell_step), then the bedrock rule of the interior cells:
+// zb = -FLT_MAX on the first step, else the cell's bedrock; where
+// z >= zb the height becomes max(max(z + exchanges, z0), zb) and the
+// bedrock max(z0, height), else the height stays (and on the first step the
+// bedrock becomes zb).  The bedrock plane is read and written in place, by
+// its own cell only, so each column has one owner: strip s writes columns
+// 62 s .. 62 s + 61 (the last strip, moved left to end at column nx - 1,
+// only those past the previous strip's), and chunks do not overlap.
+template <int P>
+__global__ void __launch_bounds__(256) k_thermal_auto_stream(const float *__restrict__ z,
+                                                             float *__restrict__ zn,
+                                                             int   nx,
+                                                             int   ny,
+                                                             float ts,
+                                                             const float *__restrict__ talus,
+                                                             float *__restrict__ bed,
+                                                             const float *__restrict__ zinit,
+                                                             int   first,
+                                                             int   strips,
+                                                             int   chunk_rows,
+                                                             int   n_items)
+{
+  const int item = th_wave_item();
+  if (item >= n_items) return;
+  const int      lane = threadIdx.x & 63;
+  const int      strip = item % strips, chunk = item / strips;
+  const int      x0 = nx > TH_OUT ? min(strip * TH_OUT - 1, nx - TH_OUT - 1) : -1;
+  const int      gx = x0 + lane;
+  const uint32_t off = (uint32_t)min(max(gx, 0), nx - 1); // column loaded
+  const uint32_t soff = (uint32_t)max(gx, 0);              // column stored
+  const bool     writer =
+      lane >= 1 && lane <= TH_OUT && gx < nx && gx >= strip * TH_OUT; // the column's owner
+  const bool     c0 = gx == 0, cn = gx == nx - 1;
+  const bool     edge_strip = x0 < 0 || x0 + TH_OUT + 1 >= nx;
+  const int      j0 = chunk * chunk_rows;
+  const int      j1 = min(j0 + chunk_rows, ny);
+  auto           load = [&](const float *p, int r) {
+    return at_off(p + (size_t)min(max(r, 0), ny - 1) * (size_t)nx, off);
+  };
+  auto load_t = [&](int r) { return talus ? load(talus, r) : ts; };
+
+  constexpr int U = P + 3; // rows r-1 .. r+P+1
+  float         R[U], TR[U];
+#pragma unroll
+  for (int q = 0; q <= P + 1; q++) R[q] = load(z, j0 - 1 + q);
+  TR[0] = ts; // row j0 - 1: no step of it here
+#pragma unroll
+  for (int q = 1; q <= P + 1; q++) TR[q] = load_t(j0 - 1 + q);
+
+  for (int r0 = j0; r0 < j1; r0 += U)
+  {
+#pragma unroll
+    for (int q = 0; q < U; q++)
+    {
+      const int r = r0 + q;
+      if (r >= j1) break;
+      const float zm = R[q % U], zc = R[(q + 1) % U], z1 = R[(q + 2) % U];
+      const float t = TR[(q + 1) % U];
+      if (r + P + 1 <= j1)
+      {
+        R[(q + P + 2) % U] = load(z, r + P + 1);
+        TR[(q + P + 2) % U] = load_t(r + P + 1);
+      }
+      float       *brow = bed + (size_t)r * (size_t)nx;
+      const float  zb = first ? -FLT_MAX : at_off((const float *)brow, off);
+      const float  zi = load(zinit, r);
+      const float  step = th_cell_step(zm, zc, z1, t);
+      const bool   up = zc >= zb;
+      float        out = up ? fmaxf(fmaxf(step, zi), zb) : zc;
+      out = th_border(out, zm, zc, z1, r, ny, edge_strip, c0, cn);
+      const bool interior = !c0 && !cn && r != 0 && r != ny - 1;
+      if (writer)
+      {
+        at_off(zn + (size_t)r * (size_t)nx, soff) = out;
+        if (interior && (up || first)) at_off(brow, soff) = up ? fmaxf(zi, out) : zb;
+      }
+    }
+  }
+}
+
+template <int P>
+static int launch_thermal_auto_stream(const float *a,
+                                      float       *b,
+                                      int          nx,
+                                      int          ny,
+                                      float        ts,
+                                      const float *talus,
+                                      float       *bed,
+                                      const float *zinit,
+                                      bool         first,
+                                      int          num_cus,
+                                      hipStream_t  st)
+{
+  static const int blocks_per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_auto_stream<P>, 256, 0) !=
+            hipSuccess ||
+        n < 1)
+    {
+      (void)hipGetLastError();
+      n = 2;
+    }
+    return n;
+  }();
+  const long strips = (nx + TH_OUT - 1) / TH_OUT;
+  const long slots = 2 * 4L * blocks_per_cu * (num_cus > 0 ? num_cus : 256);
+  long       ch = (strips * ny + slots - 1) / slots;
+  ch = ch < 3 ? 3 : ch;
+  const long chunks = (ny + ch - 1) / ch;
+  const long items = strips * chunks;
+  const int  G = (int)((items + 3) / 4);
+  k_thermal_auto_stream<P><<<G, 256, 0, st>>>(a, b, nx, ny, ts, talus, bed, zinit, first ? 1 : 0,
+                                              (int)strips, (int)ch, (int)items);
+  return check_launch("thermal_auto_stream");
+}
+
 // Kernel choice for uniform talus: 8, the LDS-blocked kernel (8 steps per
 // launch) below 2^18 cells and the streaming kernels, two steps per launch,
 // from there; 2, the streaming kernels, two steps per launch, at every
@@ -934,18 +1055,27 @@ int launch_thermal_auto(float       *z,
                         const float *talus,
                         float        talus_scalar,
                         int          iterations,
-                        hipStream_t  st)
+                        hipStream_t  st,
+                        int          num_cus)
 {
   const size_t bytes = (size_t)nx * ny * sizeof(float);
   HMG_HIP(hipMemcpyAsync(z0buf, z, bytes, hipMemcpyDeviceToDevice, st));
   float *a = z, *b = zscratch;
   dim3   blk(64, 4);
   dim3   grd((nx + 63) / 64, (ny + 3) / 4);
+  // the streaming step from 4 x 4 (one thread per cell below)
+  const bool stream = nx >= 4 && ny >= 4;
   for (int it = 0; it < iterations; it++)
   {
-    k_thermal_auto_step<<<grd, blk, 0, st>>>(a, b, nx, ny, talus, talus_scalar, bed, z0buf,
-                                             it == 0 ? 1 : 0);
-    HMG_TRY(check_launch("thermal_auto_step"));
+    if (stream)
+      HMG_TRY((launch_thermal_auto_stream<2>(a, b, nx, ny, talus_scalar, talus, bed, z0buf,
+                                             it == 0, num_cus, st)));
+    else
+    {
+      k_thermal_auto_step<<<grd, blk, 0, st>>>(a, b, nx, ny, talus, talus_scalar, bed, z0buf,
+                                               it == 0 ? 1 : 0);
+      HMG_TRY(check_launch("thermal_auto_step"));
+    }
     float *t = a;
     a = b;
     b = t;
@@ -1121,7 +1251,7 @@ static int thermal_host_call(hmg_ctx     *ctx,
     DevBuf bed;
     HMG_TRY(bed.get(ctx->pool, bytes));
     HMG_TRY(launch_thermal_auto(zb.as<float>(), tmp.as<float>(), bed.as<float>(), z0.as<float>(),
-                                nx, ny, pt, talus_scalar, iterations, st));
+                                nx, ny, pt, talus_scalar, iterations, st, ctx->num_cus));
     HMG_TRY(launch_extrapolate_borders(zb.as<float>(), nx, ny, st));
     HMG_HIP(hipStreamSynchronize(st)); // bed returns to the pool here
   }
@@ -1200,7 +1330,7 @@ int hmg_thermal_auto_bedrock_dev(hmg_ctx     *ctx,
   HMG_TRY(bed.get(ctx->pool, bytes));
   HMG_TRY(z0.get(ctx->pool, bytes));
   HMG_TRY(launch_thermal_auto(z_dev, tmp.as<float>(), bed.as<float>(), z0.as<float>(), nx, ny,
-                              talus_dev, talus_scalar, iterations, ctx->stream));
+                              talus_dev, talus_scalar, iterations, ctx->stream, ctx->num_cus));
   HMG_TRY(launch_extrapolate_borders(z_dev, nx, ny, ctx->stream));
   HMG_HIP(hipStreamSynchronize(ctx->stream));
   return HMG_OK;

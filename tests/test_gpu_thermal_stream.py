@@ -159,3 +159,19 @@ def test_plane_stream_special_values(hm, orc, streaming):
         want = orc.thermal(z0, t, it)
         same = (z.view(np.uint32) == want.view(np.uint32)) | (np.isnan(z) & np.isnan(want))
         assert same.all(), (it, np.argwhere(~same)[:5])
+
+
+@pytest.mark.parametrize("nx,ny", SHAPES)
+@pytest.mark.parametrize("iters", [1, 2, 5])
+def test_auto_bedrock_stream_matches_oracle(hm, orc, nx, ny, iters):
+    """thermal_auto_bedrock's streaming step (from 4 x 4; the bedrock plane
+    updated in place by the owner of each column, the moved last strip
+    writing only the columns past the previous strip's) with a talus plane
+    and with a scalar."""
+    z0 = _input(orc, nx, ny, nx * 7 + ny)
+    t = _plane(nx, ny, nx * ny, 3.0 / nx)
+    for talus in (t, 1.5 / nx, -1e-3):
+        z = z0.copy()
+        hm.thermal_auto_bedrock(z, talus, iters)
+        want = orc.thermal_auto_bedrock(z0, talus, iters)
+        assert _bits_equal(z, want), (np.isscalar(talus), int((z != want).sum()))
