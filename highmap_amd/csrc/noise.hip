@@ -499,7 +499,9 @@ __global__ void __launch_bounds__(256) k_noise_fbm(float *__restrict__ out,
   if (valid) out[idx] = n;
 }
 
-template <int TYPE>
+// WAVE as in k_noise_fbm: no warping planes, the lattice values shared
+// across each wavefront (64 cells of one row).
+template <int TYPE, bool WAVE = false>
 __global__ void __launch_bounds__(256) k_noise(float *__restrict__ out,
                                                const float *__restrict__ nxp,
                                                const float *__restrict__ nyp,
@@ -507,11 +509,24 @@ __global__ void __launch_bounds__(256) k_noise(float *__restrict__ out,
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.nx || j >= a.ny) return;
-  size_t idx = (size_t)j * a.nx + i;
-  float  x, y;
-  noise_xy(i, j, a, nxp ? nxp[idx] : 0.f, nyp ? nyp[idx] : 0.f, x, y);
-  out[idx] = base_noise<TYPE>(x, y, a.fseed, a.per_x, a.per_y);
+  if (j >= a.ny) return; // the whole wavefront
+  if (!WAVE && i >= a.nx) return;
+  const bool   valid = i < a.nx;
+  const size_t idx = (size_t)j * a.nx + i;
+  float        x, y, v;
+  if constexpr (WAVE)
+  {
+    const int lane = threadIdx.x & 63;
+    const int last = min(63, a.nx - 1 - (int)(blockIdx.x * blockDim.x)); // uniform
+    noise_xy(i, j, a, 0.f, 0.f, x, y);
+    v = base_noise_wave<TYPE>(x, y, a.fseed, a.per_x, a.per_y, lane, last);
+  }
+  else
+  {
+    noise_xy(i, j, a, nxp ? nxp[idx] : 0.f, nyp ? nyp[idx] : 0.f, x, y);
+    v = base_noise<TYPE>(x, y, a.fseed, a.per_x, a.per_y);
+  }
+  if (valid) out[idx] = v;
 }
 
 static NoiseArgs make_noise_args(int          nx,
@@ -610,7 +625,14 @@ int launch_noise(float       *out,
   const NoiseArgs a = make_noise_args(nx, ny, kx, ky, seed, 1, 0.f, 0.f, 0.f, bbox4, period2);
   dim3 blk(64, 4);
   dim3 grd((nx + 63) / 64, (ny + 3) / 4);
-#define HMG_NOISE(T) k_noise<T><<<grd, blk, 0, st>>>(out, noise_x, noise_y, a)
+  // without warping planes, the lattice values are shared across each
+  // wavefront (as in noise_fbm)
+  const bool wave = !noise_x && !noise_y;
+#define HMG_NOISE(T)                                                                   \
+  if (wave)                                                                            \
+    k_noise<T, true><<<grd, blk, 0, st>>>(out, nullptr, nullptr, a);                   \
+  else                                                                                 \
+    k_noise<T><<<grd, blk, 0, st>>>(out, noise_x, noise_y, a)
   switch (noise_type)
   {
   case 1: HMG_NOISE(1); break;
@@ -621,7 +643,7 @@ int launch_noise(float       *out,
   case 7: HMG_NOISE(7); break;
   case 9: HMG_NOISE(9); break;
   case 10: HMG_NOISE(10); break;
-  default: HMG_NOISE(0); break;
+  default: k_noise<0><<<grd, blk, 0, st>>>(out, noise_x, noise_y, a); break;
   }
 #undef HMG_NOISE
   return check_launch("noise");

@@ -90,3 +90,8 @@ def test_noise_fbm_shared_lattice(hm, orc, noise_type, case):
     g = hm.noise_fbm(noise_type, (nx, ny), case["kw"], 9, ctrl_param=ctrl, **kw)
     o = orc.noise_fbm(nx, ny, case["kw"], 9, ctrl=ctrl, noise_type=noise_type, **kw)
     assert np.array_equal(g, o, equal_nan=True), (case, np.nanmax(np.abs(g - o)))
+    # gpu::noise (one octave, the same wavefront forms)
+    g1 = hm.noise(noise_type, (nx, ny), case["kw"], 9, bbox=case["bbox"], period=case["period"])
+    o1 = orc.noise(nx, ny, case["kw"], 9, noise_type=noise_type, bbox=case["bbox"],
+                   period=case["period"])
+    assert np.array_equal(g1, o1, equal_nan=True), (case, "noise")
