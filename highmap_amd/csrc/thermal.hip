@@ -330,8 +330,9 @@ int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st)
 // m = max(0.2 (|other - self| - c T), 0) (divided by 1.414 on diagonals):
 // m > 0 exactly when |d| > c T (denormals kept, so a - b > 0 iff a > b), and
 // where the reference adds +0 this may add -0, which leaves the running sum
-// unchanged (the sum starts as 0 - term and is never -0).  Talus must be
-// >= 0 (the host checks; other values take k_thermal_step).
+// unchanged (the sum starts as 0 - term and is never -0).  The pair form
+// needs one talus >= 0 for every cell (the host checks): a talus plane or a
+// negative scalar takes the PLANE form, eight exchanges per cell.
 //
 // EXTRAP: the step is the last of a thermal() call, and extrapolate_borders
 // (boundary.cpp:19-41) is applied in the same launch: border columns of the
