@@ -175,3 +175,19 @@ def test_auto_bedrock_stream_matches_oracle(hm, orc, nx, ny, iters):
         hm.thermal_auto_bedrock(z, talus, iters)
         want = orc.thermal_auto_bedrock(z0, talus, iters)
         assert _bits_equal(z, want), (np.isscalar(talus), int((z != want).sum()))
+
+
+def test_plane_and_auto_bedrock_default_path_large_grid(hm, orc):
+    """Default kernel choice at 2^20 cells: a talus plane (two steps per
+    launch, an odd last step alone) and thermal_auto_bedrock's stream."""
+    nx, ny = 1030, 777
+    z0 = orc.fbm_terrain(nx, ny, 12)
+    t = _plane(nx, ny, 5, 2.0 / nx)
+    for it in (1, 4, 5):
+        z = z0.copy()
+        hm.thermal(z, t, it)
+        assert _bits_equal(z, orc.thermal(z0, t, it)), it
+    for it in (1, 3):
+        z = z0.copy()
+        hm.thermal_auto_bedrock(z, t, it)
+        assert _bits_equal(z, orc.thermal_auto_bedrock(z0, t, it)), it
