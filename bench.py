@@ -998,18 +998,21 @@ def vp_pattern_us(torch, nx, ny, storage, chunk_rows, reps=30):
     fused vpipes iteration with the kernel's decomposition (strips of 64
     lanes, 60 written; chunks of `chunk_rows` rows; four wavefronts per
     workgroup in row lockstep) and no erosion arithmetic, on planes of the
-    storage's element size."""
+    storage's element size; FP32 state with the heights and water depths in
+    one float2 plane, as the FP32 sessions keep them."""
     import ctypes as C
 
     L = probe_lib()
     L.hmg_probe_vp_pattern.argtypes = [C.c_int] + [C.c_void_p] * 8 + [C.c_int] * 4 + [
         C.POINTER(C.c_float)]
     L.hmg_probe_vp_pattern.restype = C.c_int
-    elem = 4 if storage == "f32" else 2
-    dt = torch.float32 if elem == 4 else torch.float16
+    elem = 8 if storage == "f32" else 2
+    dt = torch.float32 if elem == 8 else torch.float16
     n = nx * ny
-    z, zo = (torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(2))
-    D, Do, S, So = (torch.zeros(n, dtype=dt, device="cuda") for _ in range(4))
+    z, zo = (torch.zeros(2 * n if elem == 8 else n, dtype=torch.float32, device="cuda")
+             for _ in range(2))
+    D, Do = (torch.zeros(1 if elem == 8 else n, dtype=dt, device="cuda") for _ in range(2))
+    S, So = (torch.zeros(n, dtype=dt, device="cuda") for _ in range(2))
     F, Fo = (torch.zeros(4 * n, dtype=dt, device="cuda") for _ in range(2))
     torch.cuda.synchronize()
     ms = C.c_float()

@@ -140,7 +140,9 @@ struct PatRow
   Q     f;
 };
 
-template <class T, class Q>
+// ZD: z and zo hold float2 (height, water depth) as the FP32 sessions keep
+// them; D and Do are unused.
+template <class T, class Q, bool ZD = false>
 __global__ void __launch_bounds__(64 * PAT_WPB) k_vp_pattern(const float *__restrict__ z,
                                                             const T *__restrict__ D,
                                                             const Q *__restrict__ F,
@@ -165,9 +167,16 @@ __global__ void __launch_bounds__(64 * PAT_WPB) k_vp_pattern(const float *__rest
   const int  gi = pat_clamp(gx, nx), gi1 = pat_clamp(gx + 1, nx);
   const bool writer = lane >= 2 && lane < 62 && gx < nx;
   const int  j0 = chunk * chunk_rows, j1 = min(j0 + chunk_rows, ny);
+  typedef float v2f __attribute__((ext_vector_type(2)));
   auto load = [&](int r) {
     const uint32_t o = (uint32_t)(pat_clamp(r, ny) * nx + gi);
-    return PatRow<T, Q>{pat_ld_nt(z, o), pat_ld_nt(D, o), pat_ld_nt(F, o)};
+    if constexpr (ZD)
+    {
+      const v2f v = pat_ld_nt(reinterpret_cast<const v2f *>(z), o);
+      return PatRow<T, Q>{v.x, __builtin_bit_cast(T, v.y), pat_ld_nt(F, o)};
+    }
+    else
+      return PatRow<T, Q>{pat_ld_nt(z, o), pat_ld_nt(D, o), pat_ld_nt(F, o)};
   };
   PatRow<T, Q> R[5];
   R[0] = load(j0 - 1);
@@ -193,8 +202,14 @@ __global__ void __launch_bounds__(64 * PAT_WPB) k_vp_pattern(const float *__rest
         const uint32_t c = (uint32_t)(r * nx + gx);
         // the loaded values pass through with one operation each so that
         // every load is consumed; no arithmetic of the erosion step
-        pat_st_nt(zo, c, a0.z + (am1.z - ap1.z));
-        pat_st_nt(Do, c, a0.d);
+        if constexpr (ZD)
+          pat_st_nt(reinterpret_cast<v2f *>(zo), c,
+                    v2f{a0.z + (am1.z - ap1.z), __builtin_bit_cast(float, a0.d)});
+        else
+        {
+          pat_st_nt(zo, c, a0.z + (am1.z - ap1.z));
+          pat_st_nt(Do, c, a0.d);
+        }
         pat_st_nt(Fo, c, a0.f);
         So[c] = (T)(s00 ^ s10 ^ s01 ^ s11);
       }
@@ -250,9 +265,11 @@ int hmg_probe_stream(int         mode,
 
 // The fused vpipes iteration's memory operations (k_vp_pattern) over an
 // nx x ny grid with chunks of `chunk_rows` rows; `elem` is the state
-// storage's element size (4: FP32, 2: BF16/FP16).  The planes are the
-// caller's: z and zo nx*ny floats, D, S, Do, So nx*ny elements, F and Fo
-// nx*ny flux quadruples (4 elements each).  Runs two warm-up launches and
+// storage's element size (4: FP32, 2: BF16/FP16), or 8 for FP32 state with
+// the heights and water depths in one float2 plane (the sessions' FP32
+// layout).  The planes are the caller's: z and zo nx*ny floats (nx*ny
+// float2 for elem 8), D, S, Do, So nx*ny elements (D, Do unused for elem 8),
+// F and Fo nx*ny flux quadruples (4 elements each).  Runs two warm-up launches and
 // then `reps` on the null stream (inputs and outputs swap between launches)
 // and returns their average time in *ms.  Returns 0 or a hipError_t.
 int hmg_probe_vp_pattern(int    elem,
@@ -270,7 +287,8 @@ int hmg_probe_vp_pattern(int    elem,
                          int    reps,
                          float *ms)
 {
-  if ((elem != 4 && elem != 2) || nx < 3 || ny < 3 || chunk_rows < 1 || reps < 1 || !ms ||
+  if ((elem != 4 && elem != 2 && elem != 8) || nx < 3 || ny < 3 || chunk_rows < 1 || reps < 1 ||
+      !ms ||
       (long long)nx * ny > (1ll << 28))
     return (int)hipErrorInvalidValue;
   const int  strips = (nx + PAT_OUT - 1) / PAT_OUT;
@@ -282,9 +300,13 @@ int hmg_probe_vp_pattern(int    elem,
     const bool sw = r & 1; // ping-pong: the previous launch's outputs are the inputs
     void *iz = sw ? zo : z, *iD = sw ? Do : D, *iF = sw ? Fo : F, *iS = sw ? So : S;
     void *oz = sw ? z : zo, *oD = sw ? D : Do, *oF = sw ? F : Fo, *oS = sw ? S : So;
-    if (elem == 4)
+    typedef uint32_t q4 __attribute__((ext_vector_type(4)));
+    if (elem == 8)
+      k_vp_pattern<uint32_t, q4, true><<<G, 64 * PAT_WPB>>>(
+          (const float *)iz, nullptr, (const q4 *)iF, (const uint32_t *)iS, (float *)oz, nullptr,
+          (q4 *)oF, (uint32_t *)oS, nx, ny, strips, chunk_rows, (int)items);
+    else if (elem == 4)
     {
-      typedef uint32_t q4 __attribute__((ext_vector_type(4)));
       k_vp_pattern<uint32_t, q4><<<G, 64 * PAT_WPB>>>(
           (const float *)iz, (const uint32_t *)iD, (const q4 *)iF, (const uint32_t *)iS, (float *)oz,
           (uint32_t *)oD, (q4 *)oF, (uint32_t *)oS, nx, ny, strips, chunk_rows, (int)items);

@@ -24,14 +24,26 @@ namespace hmg
 struct StF32
 {
   using T = float;
+  // ZD: heights and water depths in one float2 plane (StF32ZD below)
+  static constexpr bool ZD = false;
   static __device__ __forceinline__ float ld(T x) { return x; }
   static __device__ __forceinline__ T     st(float x) { return x; }
   static __device__ __forceinline__ T     st_rounded(float x) { return x; }
 };
 
+// FP32 state with the heights and the water depths interleaved in one
+// float2 plane (x the height, y the depth): the fused kernels then read and
+// write one 8-byte stream per cell for both instead of two 4-byte streams.
+// The z planes a kernel receives hold float2; its D planes are unused.
+struct StF32ZD : StF32
+{
+  static constexpr bool ZD = true;
+};
+
 struct StBF16
 {
   using T = uint16_t;
+  static constexpr bool ZD = false;
   static __device__ __forceinline__ float ld(T x)
   {
     return __uint_as_float(((uint32_t)x) << 16);
@@ -68,6 +80,7 @@ struct StBF16
 struct StF16
 {
   using T = _Float16;
+  static constexpr bool ZD = false;
   static __device__ __forceinline__ float ld(T x) { return (float)x; }
   static __device__ __forceinline__ T     st(float x)
   {

@@ -142,6 +142,25 @@ int launch_thermal(hmg_ctx     *ctx,
                    bool         extrapolate = false);
 int launch_extrapolate_borders(float *z, int nx, int ny, hipStream_t st);
 
+// launch_thermal for heights held beside the water depth in float2 planes
+// (x the height, y the depth; the sessions' FP32 fused layout): thermal()
+// with a scalar talus and its extrapolate_borders on the x components of zd,
+// ping-ponging with zd_scratch (both nx*ny float2); each step carries the
+// depths over unchanged.  *result receives the plane that holds the result.
+int launch_thermal_zd(hmg_ctx     *ctx,
+                      float2      *zd,
+                      float2      *zd_scratch,
+                      int          nx,
+                      int          ny,
+                      float        talus_scalar,
+                      int          iterations,
+                      hipStream_t  st,
+                      float2     **result);
+// The x (height) or y (depth) components of n float2 into a float plane,
+// and back (the other component of zd kept).
+int launch_zd_split(const float2 *zd, float *out, int component, size_t n, hipStream_t st);
+int launch_zd_merge(float2 *zd, const float *in, int component, size_t n, hipStream_t st);
+
 // HMG_ERR_NONFINITE if the n floats at p (device) hold a NaN or an
 // infinity; waits for the context stream (vpipes.hip).
 int check_finite_f32(hmg_ctx *ctx, const float *p, size_t n);
@@ -165,6 +184,10 @@ struct hmg_session
   hmg_vpipes_params p{};
   hmg::VpConst      k{};
   size_t            esize = 4; // bytes per stored scalar
+  // zd: the FP32 fused layout, heights and water depths as one float2 plane
+  // (z[q] holds nx*ny float2: one 8-byte load and store per cell instead of
+  // two 4-byte streams; D[q] is not allocated)
+  bool              zd = false;
   hmg::DevBuf       z[2], D[2], Sp[2], F[2], rain, slots;
   hmg::DevBuf       d1, s, u, v; // plain-form intermediates
   hmg::DevBuf       zscratch;    // thermal ping-pong
@@ -195,6 +218,17 @@ namespace hmg
 // HMG_ERR_NONFINITE if the session's state holds a NaN or an infinity;
 // waits for the context stream (vpipes.hip).
 int session_check_finite(hmg_session *s);
+
+// hmg_session_create(_dev) with the heights and the water depths always in
+// separate planes (for sessions that only run thermal steps on z directly).
+int session_create_separate(hmg_ctx                 *ctx,
+                            const float             *z,
+                            bool                     z_on_dev,
+                            int                      nx,
+                            int                      ny,
+                            const hmg_vpipes_params *p,
+                            int                      storage,
+                            hmg_session            **out);
 
 int batch_advance(hmg_ctx           *ctx,
                   hmg_session *const *ss,

@@ -59,6 +59,9 @@ struct hmg_server
   std::condition_variable       cv, cv_done;
   std::deque<Job *>             queue;
   std::vector<Job *>            active;
+  // jobs taken from the queue whose sessions are being created (in neither
+  // the queue nor the active list until they join it)
+  size_t                        admitting = 0;
   std::map<long long, Job *>    jobs;
   std::thread                   worker;
   bool                          stop = false;
@@ -95,14 +98,23 @@ int admit(hmg_server *sv, Job *j)
   const hmg_job_desc &d = j->desc;
   hmg_vpipes_params  p = sv->p;
   p.mode = HMG_VP_FUSED;
+  // thermal-only jobs step their z plane directly (tick): separate planes
+  const bool thermal_only = d.kind == HMG_JOB_THERMAL;
   if (d.has_input)
+  {
+    if (thermal_only)
+      return session_create_separate(ctx, j->z_in.data(), false, d.nx, d.ny, &p, sv->storage,
+                                     &j->s);
     return hmg_session_create(ctx, j->z_in.data(), d.nx, d.ny, &p, nullptr, sv->storage, &j->s);
+  }
   // generate remap(noise_fbm(PERLIN, kw 4, seed)) on the device
   DevBuf t;
   HMG_TRY(t.get(ctx->pool, (size_t)d.nx * d.ny * sizeof(float)));
   HMG_TRY(hmg_noise_fbm_dev(ctx, t.as<float>(), d.nx, d.ny, 1, 4.f, 4.f, d.seed, 8, 0.7f, 0.5f,
                             2.f, nullptr, nullptr, nullptr, nullptr, nullptr));
   HMG_TRY(hmg_remap_dev(ctx, t.as<float>(), (size_t)d.nx * d.ny, 0.f, 1.f));
+  if (thermal_only)
+    return session_create_separate(ctx, t.as<float>(), true, d.nx, d.ny, &p, sv->storage, &j->s);
   return hmg_session_create_dev(ctx, t.as<float>(), d.nx, d.ny, &p, nullptr, sv->storage, &j->s);
 }
 
@@ -266,6 +278,7 @@ void worker_main(hmg_server *sv)
         sv->active_cells += cells;
         admitted.push_back(j);
       }
+      sv->admitting = admitted.size();
     }
     const double t0 = now_s();
     std::lock_guard<std::recursive_mutex> ctx_lock(sv->ctx->mu);
@@ -286,6 +299,7 @@ void worker_main(hmg_server *sv)
     {
       std::lock_guard<std::mutex> lk(sv->mu);
       for (Job *j : admitted) sv->active.push_back(j);
+      sv->admitting = 0;
       for (Job *j : sv->active)
         if (j->state == HMG_JOB_RUNNING && j->done < j->desc.iterations) act.push_back(j);
     }
@@ -455,7 +469,8 @@ int hmg_server_wait_all(hmg_server *sv)
 {
   if (!sv) return set_error(HMG_ERR_INVALID, "null server");
   std::unique_lock<std::mutex> lk(sv->mu);
-  sv->cv_done.wait(lk, [&] { return sv->queue.empty() && sv->active.empty(); });
+  sv->cv_done.wait(lk,
+                   [&] { return sv->queue.empty() && sv->active.empty() && sv->admitting == 0; });
   return HMG_OK;
 }
 

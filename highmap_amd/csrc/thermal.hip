@@ -524,8 +524,11 @@ __device__ __forceinline__ float th_cell_step(float zm, float z0, float z1, floa
 
 // PLANE: the exchanges per cell with its own talus (th_cell_step): a talus
 // plane (talus, 4 more bytes per cell, its rows loaded with the heights) or,
-// with talus null, a negative scalar ts.
-template <int P, bool EXTRAP, bool PLANE>
+// with talus null, a negative scalar ts.  ZD (uniform talus only): z and zn
+// are float2 planes, heights in x and water depths in y; each row load
+// brings both, and every cell written gets its depth back unchanged beside
+// its new height (16 B per cell).
+template <int P, bool EXTRAP, bool PLANE, bool ZD = false>
 __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict__ z,
                                                         float *__restrict__ zn,
                                                         int   nx,
@@ -559,22 +562,39 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
   // Ordinary loads and stores: the nontemporal hint made 4096^2 steps take
   // 42 us instead of 31 (the strip edges are read by two wavefronts, and a
   // step's output is the next step's input, held by the 256 MB cache).
-  auto load = [&](int r) {
-    const float *row = z + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
-    return at_off(row, off);
+  static_assert(!(ZD && PLANE), "the float2 layout takes a uniform talus");
+  constexpr int U = P + 3; // rows r-1 .. r+P+1
+  float         R[U];
+  float         TR[U]; // PLANE: the talus of the rows in R
+  float         DR[U]; // ZD: the water depths of the rows in R
+  // row r into ring slot q (ZD: the depth beside it)
+  auto load_to = [&](int q, int r) {
+    const size_t ro = (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
+    if constexpr (ZD)
+    {
+      const float2 v = at_off(reinterpret_cast<const float2 *>(z) + ro, off);
+      R[q] = v.x;
+      DR[q] = v.y;
+    }
+    else
+      R[q] = at_off(z + ro, off);
   };
   auto load_t = [&](int r) {
     if (!talus) return ts;
     const float *row = talus + (size_t)min(max(r, 0), ny - 1) * (size_t)nx;
     return at_off(row, off);
   };
+  // store of a height at (row r, this lane's column) with depth d (ZD)
+  auto store = [&](int r, float v, float d) {
+    if constexpr (ZD)
+      at_off(reinterpret_cast<float2 *>(zn) + (size_t)r * (size_t)nx, soff) = make_float2(v, d);
+    else
+      at_off(zn + (size_t)r * (size_t)nx, soff) = v;
+  };
 
-  constexpr int U = P + 3; // rows r-1 .. r+P+1
-  float         R[U];
-  float         TR[U]; // PLANE: the talus of the rows in R
   // pre-roll: rows j0-1 .. j0+P, and the exchanges of row j0-1 with row j0
 #pragma unroll
-  for (int q = 0; q <= P + 1; q++) R[q] = load(j0 - 1 + q);
+  for (int q = 0; q <= P + 1; q++) load_to(q, j0 - 1 + q);
   if (PLANE)
   {
     TR[0] = ts; // row j0 - 1: no step of it here
@@ -584,6 +604,7 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
   float pU = 0.f, pUR = 0.f, pUL = 0.f; // exchanges of the previous row with the row above it
   if (!PLANE) th_row_pairs(R[0], R[1], md, mdd, pU, pUR, pUL);
   float f1 = 0.f, f2 = 0.f; // EXTRAP: final values of rows r-1 and r-2
+  float d_row0 = 0.f;        // EXTRAP, ZD: the depth of row 0, written with row 2
 
   for (int r0 = j0; r0 < j1; r0 += U)
   {
@@ -593,21 +614,21 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
       const int r = r0 + q;
       if (r >= j1) break;
       const float zm = R[q % U], z0 = R[(q + 1) % U], z1 = R[(q + 2) % U];
+      const float dc = ZD ? DR[(q + 1) % U] : 0.f; // depth of row r
       // the row P + 1 below the next one, into the slot of row r - 1
       if (r + P + 1 <= j1)
       {
-        R[(q + P + 2) % U] = load(r + P + 1);
+        load_to((q + P + 2) % U, r + P + 1);
         if (PLANE) TR[(q + P + 2) % U] = load_t(r + P + 1);
       }
 
       const float step = PLANE ? th_cell_step(zm, z0, z1, TR[(q + 1) % U])
                                : th_row_step(z0, z1, md, mdd, pU, pUR, pUL);
 
-      float *row = zn + (size_t)r * (size_t)nx;
       if (!EXTRAP)
       {
         const float out = th_border(step, zm, z0, z1, r, ny, edge_strip, c0, cn);
-        if (writer) at_off(row, soff) = out;
+        if (writer) store(r, out, dc);
       }
       else
       {
@@ -623,8 +644,9 @@ __global__ void __launch_bounds__(256) k_thermal_stream(const float *__restrict_
           f = c0 ? 2.f * s1r - s2r : f;
         }
         if (r == ny - 1) f = 2.f * f1 - f2;
-        if (r == 2 && j0 == 0 && writer) at_off(zn, soff) = 2.f * f1 - f;
-        if (r != 0 && writer) at_off(row, soff) = f; // row 0 is written with row 2
+        if (r == 0) d_row0 = dc;
+        if (r == 2 && j0 == 0 && writer) store(0, 2.f * f1 - f, d_row0);
+        if (r != 0 && writer) store(r, f, dc); // row 0 is written with row 2
         f2 = f1;
         f1 = f;
       }
@@ -784,7 +806,7 @@ static int launch_thermal_stream2(const float *a,
   return check_launch("thermal_stream2");
 }
 
-template <int P, bool EXTRAP, bool PLANE = false>
+template <int P, bool EXTRAP, bool PLANE = false, bool ZD = false>
 static int launch_thermal_stream(const float *a,
                                  float       *b,
                                  int          nx,
@@ -796,8 +818,8 @@ static int launch_thermal_stream(const float *a,
 {
   static const int blocks_per_cu = [] {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream<P, EXTRAP, PLANE>, 256,
-                                                     0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_thermal_stream<P, EXTRAP, PLANE, ZD>,
+                                                     256, 0) !=
             hipSuccess ||
         n < 1)
     {
@@ -815,7 +837,7 @@ static int launch_thermal_stream(const float *a,
   const long chunks = (ny + ch - 1) / ch;
   const long items = strips * chunks;
   const int  G = (int)((items + 3) / 4);
-  k_thermal_stream<P, EXTRAP, PLANE>
+  k_thermal_stream<P, EXTRAP, PLANE, ZD>
       <<<G, 256, 0, st>>>(a, b, nx, ny, ts, talus, (int)strips, (int)ch, (int)items);
   return check_launch("thermal_stream");
 }
@@ -1041,6 +1063,103 @@ int launch_thermal(hmg_ctx     *ctx,
     *result = a;
   else if (a != z)
     HMG_HIP(hipMemcpyAsync(z, a, (size_t)nx * ny * sizeof(float), hipMemcpyDeviceToDevice, st));
+  return HMG_OK;
+}
+
+// ------------------------------------------- heights beside water depths
+//
+// Sessions of FP32 state in the fused form keep the heights and the water
+// depths as one float2 plane (one 8-byte stream each way per cell in the
+// hydraulic kernel).  Thermal steps on them stream the float2 rows
+// (k_thermal_stream<2, ·, false, true>), carrying the depths over; other
+// cases split the heights out, run launch_thermal and merge them back.
+
+__global__ void __launch_bounds__(256) k_zd_split(const float2 *__restrict__ zd,
+                                                  float *__restrict__ out,
+                                                  int    component,
+                                                  size_t n)
+{
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+  {
+    const float2 v = zd[c];
+    out[c] = component ? v.y : v.x;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_zd_merge(float2 *__restrict__ zd,
+                                                  const float *__restrict__ in,
+                                                  int    component,
+                                                  size_t n)
+{
+  for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+  {
+    float2 v = zd[c];
+    if (component)
+      v.y = in[c];
+    else
+      v.x = in[c];
+    zd[c] = v;
+  }
+}
+
+int launch_zd_split(const float2 *zd, float *out, int component, size_t n, hipStream_t st)
+{
+  const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  if (nb > 0) k_zd_split<<<nb, 256, 0, st>>>(zd, out, component, n);
+  return check_launch("zd_split");
+}
+
+int launch_zd_merge(float2 *zd, const float *in, int component, size_t n, hipStream_t st)
+{
+  const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  if (nb > 0) k_zd_merge<<<nb, 256, 0, st>>>(zd, in, component, n);
+  return check_launch("zd_merge");
+}
+
+int launch_thermal_zd(hmg_ctx     *ctx,
+                      float2      *zd,
+                      float2      *zd_scratch,
+                      int          nx,
+                      int          ny,
+                      float        talus_scalar,
+                      int          iterations,
+                      hipStream_t  st,
+                      float2     **result)
+{
+  const int cus = ctx && ctx->num_cus > 0 ? ctx->num_cus : 256;
+  // the pair form needs a talus >= 0 (a NaN fails the test)
+  if (talus_scalar >= 0.f && nx >= 4 && ny >= 4 && iterations > 0)
+  {
+    float2 *a = zd, *b = zd_scratch;
+    for (int done = 0; done < iterations; done++)
+    {
+      const float *ia = reinterpret_cast<const float *>(a);
+      float       *ob = reinterpret_cast<float *>(b);
+      if (done == iterations - 1)
+        HMG_TRY((launch_thermal_stream<2, true, false, true>(ia, ob, nx, ny, talus_scalar, cus,
+                                                             st)));
+      else
+        HMG_TRY((launch_thermal_stream<2, false, false, true>(ia, ob, nx, ny, talus_scalar, cus,
+                                                              st)));
+      std::swap(a, b);
+    }
+    *result = a;
+    return HMG_OK;
+  }
+  const size_t n = (size_t)nx * ny;
+  DevBuf       h, hs;
+  HMG_TRY(h.get(ctx->pool, n * 4));
+  HMG_TRY(hs.get(ctx->pool, n * 4));
+  HMG_TRY(launch_zd_split(zd, h.as<float>(), 0, n, st));
+  float *res = nullptr;
+  HMG_TRY(launch_thermal(ctx, h.as<float>(), hs.as<float>(), nx, ny, nullptr, talus_scalar,
+                         iterations, nullptr, st, &res, true));
+  HMG_TRY(launch_zd_merge(zd, res, 0, n, st));
+  // the pool planes return when this call ends: wait for the merge first
+  HMG_HIP(hipStreamSynchronize(st));
+  *result = zd;
   return HMG_OK;
 }
 

@@ -348,8 +348,18 @@ __device__ __forceinline__ RowA stream_load(const float *__restrict__ z,
 {
   RowA     a;
   Flux4<S> f = ld_stream(F, row_off);
-  a.z = ld_stream(z, row_off);
-  a.dr = S::ld(ld_stream(D, row_off));
+  if constexpr (S::ZD)
+  {
+    // the height and the water depth in one 8-byte load
+    const float2 zd = ld_stream(reinterpret_cast<const float2 *>(z), row_off);
+    a.z = zd.x;
+    a.dr = zd.y;
+  }
+  else
+  {
+    a.z = ld_stream(z, row_off);
+    a.dr = S::ld(ld_stream(D, row_off));
+  }
   a.fl = S::ld(f.l);
   a.fr = S::ld(f.r);
   a.ft = S::ld(f.t);
@@ -479,6 +489,7 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
   // issued one row before they are consumed, so every load has a whole row
   // step to arrive; six A rows and one row of gathered texels are live.
   static_assert(!GA || (PF && !FLOW), "gathers ahead imply the load-ahead row");
+  static_assert(!(S::ZD && FLOW), "flow_simulation keeps separate planes");
   constexpr int  U = GA ? 6 : (PF ? 5 : 4);
   constexpr bool ROW_SYNC = !NOSYNC && !std::is_same<S, StBF16>::value;
   RowA       R[U];
@@ -606,9 +617,12 @@ __device__ __forceinline__ void vp_stream_item(const float *__restrict__ z,
         // the caches for the gathers.  At 4096^2 FP32 this takes a launch
         // from 226 to 212 us; the hint on the loads alone makes it 235, on
         // every store (the sediment's too) 229.
-        st_stream(zo, c, zn);
+        if constexpr (S::ZD)
+          st_stream(reinterpret_cast<float2 *>(zo), c, make_float2(zn, d2));
+        else
+          st_stream(zo, c, zn);
         at_off(So, c) = S::st(sn);
-        st_stream(Do, c, S::st_rounded(d2));
+        if constexpr (!S::ZD) st_stream(Do, c, S::st_rounded(d2));
         Flux4<S> o;
         o.l = S::st_rounded(c_0[0]);
         o.r = S::st_rounded(c_0[1]);
@@ -946,14 +960,16 @@ static Fix128 *slot_set(hmg_session *s, long it)
   return s->slots.as<Fix128>() + (size_t)(((it % 3) + 3) % 3) * NSLOT;
 }
 
+// D0: the initial water depth plane (the session's D[0], or for the float2
+// layout a plane the caller then merges into z[0]).
 template <class S>
-static int session_init_state(hmg_session *s)
+static int session_init_state(hmg_session *s, void *D0)
 {
   size_t      n = (size_t)s->nx * s->ny;
   hipStream_t st = s->ctx->stream;
   int         nb = (int)((n + 255) / 256);
   if (nb > 4096) nb = 4096;
-  k_fill_state<S><<<nb, 256, 0, st>>>((typename S::T *)s->D[0].ptr,
+  k_fill_state<S><<<nb, 256, 0, st>>>((typename S::T *)D0,
                                       (typename S::T *)s->Sp[0].ptr,
                                       (Flux4<S> *)s->F[0].ptr,
                                       s->rain.as<float>(),
@@ -964,8 +980,7 @@ static int session_init_state(hmg_session *s)
   HMG_TRY(check_launch("zero_fix"));
   // W0 = sum(d) and R = sum(rain) (hydraulic_vpipes_gpu.cpp:54-55), exact
   Fix128 *w_set = s->slots.as<Fix128>() + 3 * NSLOT;
-  k_vp_sum<S><<<nb, 256, 0, st>>>((const typename S::T *)s->D[0].ptr, n, 1.f, 0, w_set,
-                                  nullptr);
+  k_vp_sum<S><<<nb, 256, 0, st>>>((const typename S::T *)D0, n, 1.f, 0, w_set, nullptr);
   HMG_TRY(check_launch("vp_sum(W0)"));
   Fix128 h[NSLOT];
   HMG_HIP(hipMemcpyAsync(h, w_set, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -1131,11 +1146,39 @@ static int session_export(hmg_session *s, float *d, float *sd, float *u, float *
   hipStream_t st = s->ctx->stream;
   dim3        blk(64, 4), grd((nx + 63) / 64, (ny + 3) / 4);
   VpDyn       y = make_dyn(s);
-  k_vp_export<S><<<grd, blk, 0, st>>>((const Flux4<S> *)s->F[s->cur].ptr,
-                                      (const T *)s->D[s->cur].ptr,
+  const void *Dp = s->D[s->cur].ptr;
+  DevBuf      Dt; // float2 layout: the depths split out
+  if (s->zd)
+  {
+    const size_t n = (size_t)nx * ny;
+    HMG_TRY(Dt.get(s->ctx->pool, n * 4));
+    HMG_TRY(launch_zd_split((const float2 *)s->z[s->cur].ptr, Dt.as<float>(), 1, n, st));
+    Dp = Dt.ptr;
+  }
+  k_vp_export<S><<<grd, blk, 0, st>>>((const Flux4<S> *)s->F[s->cur].ptr, (const T *)Dp,
                                       (const T *)s->Sp[s->cur].ptr, s->rain.as<float>(), d,
                                       sd, u, v, slot_set(s, s->iter - 1), s->k, y);
-  return check_launch("vp_export");
+  HMG_TRY(check_launch("vp_export"));
+  // the split plane returns to the pool when this call ends
+  if (s->zd) HMG_HIP(hipStreamSynchronize(st));
+  return HMG_OK;
+}
+
+// The session's current heights as a float plane: its z plane, or for the
+// float2 layout the heights split into `tmp` (n floats, from the pool).
+static int session_heights(hmg_session *s, DevBuf &tmp, const float **out)
+{
+  if (!s->zd)
+  {
+    *out = s->z[s->cur].as<float>();
+    return HMG_OK;
+  }
+  const size_t n = (size_t)s->nx * s->ny;
+  HMG_TRY(tmp.get(s->ctx->pool, n * 4));
+  HMG_TRY(launch_zd_split((const float2 *)s->z[s->cur].ptr, tmp.as<float>(), 0, n,
+                          s->ctx->stream));
+  *out = tmp.as<float>();
+  return HMG_OK;
 }
 
 static int dispatch_step(hmg_session *s)
@@ -1144,7 +1187,7 @@ static int dispatch_step(hmg_session *s)
   {
   case HMG_STORAGE_BF16: return session_step<StBF16>(s);
   case HMG_STORAGE_F16: return session_step<StF16>(s);
-  default: return session_step<StF32>(s);
+  default: return s->zd ? session_step_fused<StF32ZD>(s) : session_step<StF32>(s);
   }
 }
 
@@ -1175,7 +1218,7 @@ static int stream_blocks_per_cu(const hmg_session *s, int variant)
 {
   if (s->storage == HMG_STORAGE_BF16) return stream_blocks_per_cu_t<StBF16>(s, variant);
   if (s->storage == HMG_STORAGE_F16) return stream_blocks_per_cu_t<StF16>(s, variant);
-  return stream_blocks_per_cu_t<StF32>(s, variant);
+  return s->zd ? stream_blocks_per_cu_t<StF32ZD>(s, variant) : stream_blocks_per_cu_t<StF32>(s, variant);
 }
 
 // ------------------------------------------------ online shape tuning
@@ -1361,7 +1404,8 @@ static int session_create_common(hmg_ctx                 *ctx,
                                  const float             *rain,
                                  bool                     rain_on_dev,
                                  int                      storage,
-                                 hmg_session            **out)
+                                 hmg_session            **out,
+                                 bool                     allow_zd = true)
 {
   clear_error();
   if (!out) return set_error(HMG_ERR_INVALID, "null output pointer");
@@ -1391,6 +1435,9 @@ static int session_create_common(hmg_ctx                 *ctx,
   s->p = prm;
   s->k = make_vp_const(prm, nx, ny);
   s->esize = storage == HMG_STORAGE_F32 ? 4 : 2;
+  // FP32 state in the fused form: heights and water depths in one float2
+  // plane (one 8-byte stream each way per cell instead of two 4-byte ones)
+  s->zd = allow_zd && storage == HMG_STORAGE_F32 && prm.mode == HMG_VP_FUSED;
   // Launch shape of the fused kernel: at 4096^2 the measured best is 64 rows
   // per wavefront for FP32 state and 48 for 16-bit state (loads one row
   // ahead).  Smaller grids get shorter chunks so that a launch still has
@@ -1405,13 +1452,16 @@ static int session_create_common(hmg_ctx                 *ctx,
   s->prefetch = 1;
   const size_t n = (size_t)nx * ny;
   int          rc = HMG_OK;
+  DevBuf       zt, Dt; // float2 layout: the input heights and depths before the merge
   for (int q = 0; q < 2 && rc == HMG_OK; q++)
   {
-    rc = s->z[q].get(ctx->pool, n * 4);
-    if (rc == HMG_OK) rc = s->D[q].get(ctx->pool, n * s->esize);
+    rc = s->z[q].get(ctx->pool, n * (s->zd ? 8 : 4));
+    if (rc == HMG_OK && !s->zd) rc = s->D[q].get(ctx->pool, n * s->esize);
     if (rc == HMG_OK) rc = s->Sp[q].get(ctx->pool, n * s->esize);
     if (rc == HMG_OK) rc = s->F[q].get(ctx->pool, n * s->esize * 4);
   }
+  if (rc == HMG_OK && s->zd) rc = zt.get(ctx->pool, n * 4);
+  if (rc == HMG_OK && s->zd) rc = Dt.get(ctx->pool, n * 4);
   if (rc == HMG_OK) rc = s->slots.get(ctx->pool, sizeof(Fix128) * 4 * NSLOT);
   if (rc == HMG_OK && rain) rc = s->rain.get(ctx->pool, n * 4);
   if (rc == HMG_OK && prm.mode == HMG_VP_PLAIN)
@@ -1427,7 +1477,7 @@ static int session_create_common(hmg_ctx                 *ctx,
     return rc;
   }
   hipStream_t st = ctx->stream;
-  hipError_t  e = hipMemcpyAsync(s->z[0].ptr, z, n * 4,
+  hipError_t  e = hipMemcpyAsync(s->zd ? zt.ptr : s->z[0].ptr, z, n * 4,
                                  z_on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st);
   if (e == hipSuccess && rain)
     e = hipMemcpyAsync(s->rain.ptr, rain, n * 4,
@@ -1439,9 +1489,17 @@ static int session_create_common(hmg_ctx                 *ctx,
   }
   switch (storage)
   {
-  case HMG_STORAGE_BF16: rc = session_init_state<StBF16>(s); break;
-  case HMG_STORAGE_F16: rc = session_init_state<StF16>(s); break;
-  default: rc = session_init_state<StF32>(s); break;
+  case HMG_STORAGE_BF16: rc = session_init_state<StBF16>(s, s->D[0].ptr); break;
+  case HMG_STORAGE_F16: rc = session_init_state<StF16>(s, s->D[0].ptr); break;
+  default: rc = session_init_state<StF32>(s, s->zd ? Dt.ptr : s->D[0].ptr); break;
+  }
+  if (rc == HMG_OK && s->zd)
+  {
+    rc = launch_zd_merge((float2 *)s->z[0].ptr, zt.as<float>(), 0, n, st);
+    if (rc == HMG_OK) rc = launch_zd_merge((float2 *)s->z[0].ptr, Dt.as<float>(), 1, n, st);
+    // the input planes return to the pool when this call ends
+    if (rc == HMG_OK && hipStreamSynchronize(st) != hipSuccess)
+      rc = set_error(HMG_ERR_DEVICE, "state merge failed");
   }
   if (rc != HMG_OK)
   {
@@ -1519,10 +1577,11 @@ int session_check_finite(hmg_session *s)
   k_flag_clear<<<1, 1, 0, st>>>(f);
   HMG_TRY(check_launch("flag_clear"));
   const int c = s->cur;
-  HMG_TRY(launch_nonfinite<uint32_t>(s->z[c].ptr, n, 0x7f800000u, f, st));
+  // the float2 layout: heights and depths in one plane of 2n floats
+  HMG_TRY(launch_nonfinite<uint32_t>(s->z[c].ptr, s->zd ? 2 * n : n, 0x7f800000u, f, st));
   if (s->storage == HMG_STORAGE_F32)
   {
-    HMG_TRY(launch_nonfinite<uint32_t>(s->D[c].ptr, n, 0x7f800000u, f, st));
+    if (!s->zd) HMG_TRY(launch_nonfinite<uint32_t>(s->D[c].ptr, n, 0x7f800000u, f, st));
     HMG_TRY(launch_nonfinite<uint32_t>(s->Sp[c].ptr, n, 0x7f800000u, f, st));
     HMG_TRY(launch_nonfinite<uint32_t>(s->F[c].ptr, 4 * n, 0x7f800000u, f, st));
   }
@@ -1578,6 +1637,27 @@ int hmg_session_create_dev(hmg_ctx                 *ctx,
   return session_create_common(ctx, z_dev, true, nx, ny, p, rain_dev, true, storage, out);
 }
 
+} // extern "C"
+
+namespace hmg
+{
+
+int session_create_separate(hmg_ctx                 *ctx,
+                            const float             *z,
+                            bool                     z_on_dev,
+                            int                      nx,
+                            int                      ny,
+                            const hmg_vpipes_params *p,
+                            int                      storage,
+                            hmg_session            **out)
+{
+  return session_create_common(ctx, z, z_on_dev, nx, ny, p, nullptr, false, storage, out, false);
+}
+
+} // namespace hmg
+
+extern "C" {
+
 int hmg_session_vpipes(hmg_session *s, int iterations)
 {
   clear_error();
@@ -1597,13 +1677,21 @@ int hmg_session_thermal(hmg_session *s, float talus, int iterations)
   if (iterations < 0) return set_error(HMG_ERR_INVALID, "iterations must be >= 0");
   std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
   HMG_TRY(ctx_enter(s->ctx));
+  // the other z plane is free between iterations: after an odd number of
+  // launches the result is there, and the two planes trade places
+  if (s->zd)
+  {
+    float2 *zc = (float2 *)s->z[s->cur].ptr, *res = zc;
+    HMG_TRY(launch_thermal_zd(s->ctx, zc, (float2 *)s->z[s->cur ^ 1].ptr, s->nx, s->ny, talus,
+                              iterations, s->ctx->stream, &res));
+    if (res != zc) std::swap(s->z[s->cur].ptr, s->z[s->cur ^ 1].ptr);
+    return HMG_OK;
+  }
   float *zc = s->z[s->cur].as<float>();
   float *zt = s->z[s->cur ^ 1].as<float>();
   float *res = zc;
   HMG_TRY(launch_thermal(s->ctx, zc, zt, s->nx, s->ny, nullptr, talus, iterations, nullptr,
                          s->ctx->stream, &res, true));
-  // the other z plane is free between iterations: after an odd number of
-  // launches the result is there, and the two planes trade places
   if (res != zc) std::swap(s->z[s->cur].ptr, s->z[s->cur ^ 1].ptr);
   return HMG_OK;
 }
@@ -1617,8 +1705,13 @@ int hmg_session_export(hmg_session *s, float *z_host, float *d_host, float *s_ho
   HMG_TRY(ctx_enter(s->ctx));
   const size_t n = (size_t)s->nx * s->ny;
   hipStream_t  st = s->ctx->stream;
+  DevBuf       zh; // float2 layout: the heights split out
   if (z_host)
-    HMG_HIP(hipMemcpyAsync(z_host, s->z[s->cur].ptr, n * 4, hipMemcpyDeviceToHost, st));
+  {
+    const float *zp = nullptr;
+    HMG_TRY(session_heights(s, zh, &zp));
+    HMG_HIP(hipMemcpyAsync(z_host, zp, n * 4, hipMemcpyDeviceToHost, st));
+  }
   if (d_host || s_host || u_host || v_host)
   {
     DevBuf out;
@@ -1729,7 +1822,10 @@ int hmg_session_sums(hmg_session *s, double *sum_z, double *sum_s, double *sum_d
   HMG_TRY(check_launch("zero_fix"));
   int nb = (int)((n + 255) / 256);
   if (nb > 2048) nb = 2048;
-  const float *src[3] = {s->z[s->cur].as<float>(), o + n, o};
+  DevBuf       zh;
+  const float *zp = nullptr;
+  HMG_TRY(session_heights(s, zh, &zp));
+  const float *src[3] = {zp, o + n, o};
   for (int q = 0; q < 3; q++)
   {
     k_vp_sum<StF32><<<nb, 256, 0, st>>>(src[q], n, 1.f, 0, sets + q * NSLOT, nullptr);
@@ -1784,7 +1880,17 @@ int hmg_session_get_tuning(hmg_session *s, int *chunk_rows, int *prefetch, int *
 
 float *hmg_session_z_dev(hmg_session *s)
 {
-  return s ? s->z[s->cur].as<float>() : nullptr;
+  if (!s) return nullptr;
+  if (!s->zd) return s->z[s->cur].as<float>();
+  // float2 layout: a copy of the heights in the session's scratch plane,
+  // made in stream order (valid until the session's next call)
+  std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+  const size_t n = (size_t)s->nx * s->ny;
+  if (ctx_enter(s->ctx) != HMG_OK || s->zscratch.get(s->ctx->pool, n * 4) != HMG_OK ||
+      launch_zd_split((const float2 *)s->z[s->cur].ptr, s->zscratch.as<float>(), 0, n,
+                      s->ctx->stream) != HMG_OK)
+    return nullptr;
+  return s->zscratch.as<float>();
 }
 
 int hmg_session_sync(hmg_session *s)
@@ -2061,10 +2167,11 @@ int batch_advance(hmg_ctx           *ctx,
   if (n <= 0) return HMG_OK;
   const int  storage = ss[0]->storage;
   const bool ke1 = ss[0]->k.kexp == 1.f;
+  const bool zd = ss[0]->zd;
   for (int i = 0; i < n; i++)
   {
-    if (ss[i]->storage != storage)
-      return set_error(HMG_ERR_INVALID, "batched maps must share one storage type");
+    if (ss[i]->storage != storage || ss[i]->zd != zd)
+      return set_error(HMG_ERR_INVALID, "batched maps must share one storage type and layout");
     if ((ss[i]->k.kexp == 1.f) != ke1)
       return set_error(HMG_ERR_INVALID, "batched maps must share k_discharge_exp == 1 or != 1");
     if (iters[i] < 0) return set_error(HMG_ERR_INVALID, "negative iteration count");
@@ -2088,7 +2195,7 @@ int batch_advance(hmg_ctx           *ctx,
   {
   case HMG_STORAGE_BF16: return HMG_BATCH(StBF16);
   case HMG_STORAGE_F16: return HMG_BATCH(StF16);
-  default: return HMG_BATCH(StF32);
+  default: return zd ? HMG_BATCH(StF32ZD) : HMG_BATCH(StF32);
   }
 #undef HMG_BATCH
 }
@@ -2215,7 +2322,8 @@ static int batch_blocks_per_cu(const hmg_batch *b, int pf)
   const bool ke1 = b->maps[0]->k.kexp == 1.f;
   if (b->storage == HMG_STORAGE_BF16) return batch_blocks_per_cu_t<StBF16>(rain, ke1, pf);
   if (b->storage == HMG_STORAGE_F16) return batch_blocks_per_cu_t<StF16>(rain, ke1, pf);
-  return batch_blocks_per_cu_t<StF32>(rain, ke1, pf);
+  return b->maps[0]->zd ? batch_blocks_per_cu_t<StF32ZD>(rain, ke1, pf)
+                        : batch_blocks_per_cu_t<StF32>(rain, ke1, pf);
 }
 
 // Work items of one launch over every map of the batch at chunk length

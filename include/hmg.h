@@ -470,7 +470,11 @@ int hmg_session_max_backtrace(hmg_session *s, double *max_backtrace);
 int hmg_session_set_tuning(hmg_session *s, int chunk_rows, int prefetch);
 /* The current shape; *tuning is 1 while the online probes still run. */
 int hmg_session_get_tuning(hmg_session *s, int *chunk_rows, int *prefetch, int *tuning);
-/* Device pointer of the session's current z plane (FP32, nx*ny). */
+/* Device pointer of the session's current heights (FP32, nx*ny).  Sessions
+ * of FP32 state in the fused form keep the heights beside the water depths
+ * in one float2 plane; for them this is a copy of the heights, made on the
+ * context stream, valid until the session's next call (null if the copy
+ * cannot be made). */
 float *hmg_session_z_dev(hmg_session *s);
 int    hmg_session_sync(hmg_session *s);
 int    hmg_session_destroy(hmg_session *s);
