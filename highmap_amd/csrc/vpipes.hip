@@ -24,7 +24,8 @@
 // slots into the top-up amount.  Three slot sets rotate: launch k reads set
 // k-1, adds into set k and clears set k+1.
 //
-// Kernels (storage type S in {f32, bf16, f16}):
+// Kernels (storage type S in {f32, bf16, f16}; FP32 sessions in the fused
+// form keep z and D as one float2 plane, S = StF32ZD):
 //   k_vp_stream, k_vp_stream_ga
 //                  the fused form, one launch per iteration: each wavefront
 //                  streams a strip of 64 columns (60 written) down a chunk
