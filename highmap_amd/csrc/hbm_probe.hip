@@ -295,7 +295,8 @@ int hmg_probe_vp_pattern(int    elem,
   const int  chunks = (ny + chunk_rows - 1) / chunk_rows;
   const long items = (long)strips * chunks;
   if (items > (1l << 30)) return (int)hipErrorInvalidValue;
-  const int G = (int)((items + PAT_WPB - 1) / PAT_WPB);
+  // workgroups rounded up to a multiple of 8, as the fused kernel's launches
+  const int G = (int)(((items + PAT_WPB - 1) / PAT_WPB + 7) & ~7l);
   auto      launch = [&](int r) {
     const bool sw = r & 1; // ping-pong: the previous launch's outputs are the inputs
     void *iz = sw ? zo : z, *iD = sw ? Do : D, *iF = sw ? Fo : F, *iS = sw ? So : S;

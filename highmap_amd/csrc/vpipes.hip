@@ -396,6 +396,13 @@ __device__ __forceinline__ int xcd_wave_item()
   return __builtin_amdgcn_readfirstlane(b * WPB + (threadIdx.x >> 6));
 }
 
+// Workgroups of a launch of n_items wavefront items, rounded up to a multiple
+// of 8 so that the XCD-contiguous order above applies at every size; the
+// extra workgroups hold no item and return at once.  At 4096^2 the 70-row
+// chunks make 1018 workgroups of items: padded, an FP32 or FP16 launch is
+// about 1% faster, and 74-row chunks 5% faster.
+static inline int xcd_grid(int n_items) { return ((n_items + WPB - 1) / WPB + 7) & ~7; }
+
 // One wavefront's work item of one iteration of one map: a strip of 60
 // output columns over `chunk_rows` rows.  `item` numbers the map's items;
 // the wavefront of item 0 clears the slot set of the next iteration.
@@ -1070,7 +1077,7 @@ static int session_step_fused(hmg_session *s)
   const int   strips = (nx + STRIP_OUT - 1) / STRIP_OUT;
   const int   ch = s->chunk_rows;
   const int   n_items = strips * ((ny + ch - 1) / ch);
-  const int   G = (n_items + WPB - 1) / WPB;
+  const int   G = xcd_grid(n_items);
   VpDyn       y = make_dyn(s);
   const T    *D = (const T *)s->D[a].ptr;
   const T    *Sp = (const T *)s->Sp[a].ptr;
@@ -2109,8 +2116,8 @@ static int batch_advance_t(hmg_ctx           *ctx,
   {
     while (active > 0 && d[active - 1].iters <= t) active--;
     const int items = prefix[active];
-    const int G = (items + WPB - 1) / WPB;
-    if (G == 0) continue;
+    if (items == 0) continue;
+    const int G = xcd_grid(items);
     const VpMapDesc *tab = table.as<VpMapDesc>();
     if (any_rain)
       k_vp_batch<S, false, PF, KEXP1, GA><<<G, 64 * WPB, 0, st>>>(tab, active, items, t);
@@ -2653,7 +2660,7 @@ int hmg_flow_simulation_dev(hmg_ctx     *ctx,
   long       ch = (ny + chunks - 1) / chunks;
   ch = std::max(8L, std::min(64L, ch));
   const int n_items = strips * (int)((ny + ch - 1) / ch);
-  const int G = (n_items + WPB - 1) / WPB;
+  const int G = xcd_grid(n_items);
   int       a = 0;
   for (int it = 0; it < iterations; it++, a ^= 1)
   {
