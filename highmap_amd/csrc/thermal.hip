@@ -389,6 +389,11 @@ __device__ __forceinline__ float th_pair_diag(float self, float other, float mdd
   return copysignf(q, nd);
 }
 
+// Workgroups (4 wavefront items each) of a streaming launch, rounded up to
+// a multiple of 8 so that th_wave_item's XCD-contiguous order applies at
+// every size; the extra workgroups hold no item and return at once.
+static inline int th_grid(long items) { return (int)(((items + 3) / 4 + 7) & ~7L); }
+
 __device__ __forceinline__ int th_wave_item()
 {
   const int G = gridDim.x;
@@ -800,7 +805,7 @@ static int launch_thermal_stream2(const float *a,
   ch = ch < 3 ? 3 : ch;
   const long chunks = (ny + ch - 1) / ch;
   const long items = strips * chunks;
-  const int  G = (int)((items + 3) / 4);
+  const int  G = th_grid(items);
   k_thermal_stream2<P, PLANE>
       <<<G, 256, 0, st>>>(a, b, nx, ny, ts, talus, (int)strips, (int)ch, (int)items);
   return check_launch("thermal_stream2");
@@ -836,7 +841,7 @@ static int launch_thermal_stream(const float *a,
   ch = ch < 3 ? 3 : ch;
   const long chunks = (ny + ch - 1) / ch;
   const long items = strips * chunks;
-  const int  G = (int)((items + 3) / 4);
+  const int  G = th_grid(items);
   k_thermal_stream<P, EXTRAP, PLANE, ZD>
       <<<G, 256, 0, st>>>(a, b, nx, ny, ts, talus, (int)strips, (int)ch, (int)items);
   return check_launch("thermal_stream");
@@ -954,7 +959,7 @@ static int launch_thermal_auto_stream(const float *a,
   ch = ch < 3 ? 3 : ch;
   const long chunks = (ny + ch - 1) / ch;
   const long items = strips * chunks;
-  const int  G = (int)((items + 3) / 4);
+  const int  G = th_grid(items);
   k_thermal_auto_stream<P><<<G, 256, 0, st>>>(a, b, nx, ny, ts, talus, bed, zinit, first ? 1 : 0,
                                               (int)strips, (int)ch, (int)items);
   return check_launch("thermal_auto_stream");
